@@ -1,0 +1,244 @@
+/*
+ * spmm.h — C-ABI of libsegtaco, the MI355X (gfx950) CSR SpMM engine.
+ *
+ * Two layers:
+ *
+ * 1. The segTACO operator surface (reference gpu_kernels.cuh:4-93): entry
+ *    points `void f(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B)`
+ *    with host operands. Each call copies A and B to the device, computes
+ *    C = A * B, copies C back and frees device memory, and stores the elapsed
+ *    milliseconds of the device work (partition + kernels, the reference's
+ *    TIME_GPU scope, gpu_library.h:853-857) for get_gpu_timer_result().
+ *    Prior contents of C are ignored. The reference declares these with C++
+ *    linkage; libsegtaco exports them with C linkage (this header) and also
+ *    with C++ linkage, so objects compiled against the reference header link
+ *    unchanged.
+ *
+ * 2. A device-resident API (segtaco_*): CSR operands, B and C already in
+ *    device memory, explicit plans (schedule, vector width, lane grouping,
+ *    tile), caller-provided workspace and stream, integer status codes. The
+ *    batched and serving layers (batch.h, serve.h) are built on it.
+ *
+ * Semantics shared by both layers: C (M x N, row-major, ld >= N) is
+ * overwritten with A * B; A is M x K CSR with int32 pos[M+1] and crd[nnz]
+ * (columns in [0, K)); B is K x N row-major; a NULL vals pointer reads every
+ * stored value as 1.0 (gpu_library.h:154-161). Every schedule is free of
+ * floating-point atomics, so results are bit-identical run to run.
+ */
+#ifndef SEGTACO_SPMM_H
+#define SEGTACO_SPMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "segtaco/taco_tensor_t.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes and error mode                                               */
+/* ------------------------------------------------------------------------ */
+#define SEGTACO_OK 0
+#define SEGTACO_ERR_INVALID 1     /* bad argument, shape or CSR structure */
+#define SEGTACO_ERR_HIP 2         /* HIP runtime error */
+#define SEGTACO_ERR_NOMEM 3       /* device or host allocation failed */
+#define SEGTACO_ERR_UNSUPPORTED 4 /* plan not supported for these operands */
+#define SEGTACO_ERR_CAPACITY 5    /* request larger than a configured limit */
+#define SEGTACO_ERR_STATE 6       /* object not in a state for this call */
+
+/* Error behaviour of the operator-surface entry points, which return void.
+ * SEGTACO_ERRMODE_EXIT (default) prints the error and exits the process with
+ * a non-zero code, as the reference's gpuErrchk does (gpu_library.h:22-30).
+ * SEGTACO_ERRMODE_RECORD records the error (segtaco_last_error) and returns.
+ * Returns the previous mode. */
+#define SEGTACO_ERRMODE_EXIT 0
+#define SEGTACO_ERRMODE_RECORD 1
+int32_t segtaco_set_error_mode(int32_t mode);
+/* Status of the last failing call on this thread (0 if none), and its
+ * message. segtaco_clear_error resets both. */
+int32_t segtaco_last_error(void);
+const char *segtaco_last_error_string(void);
+void segtaco_clear_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Operator surface (reference gpu_kernels.cuh)                              */
+/* ------------------------------------------------------------------------ */
+/* gpu_kernels.cuh:4, gpu_kernels.cu:20-22: ms of the last call's device work
+ * (-1 before any call). */
+float get_gpu_timer_result(void);
+
+/* The five north-star entry points. */
+/* spmm_rb_sr.h:220-243: row-balanced, sequential reduction. */
+void spmm_rb_sr(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_rb_pr.h:333-375: row-balanced, parallel (tree) reduction. */
+void spmm_rb_pr(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_eb_sr.h:335-423: element-balanced, sequential reduction. */
+void spmm_eb_sr(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_eb_pr.h:435-480: element-balanced, segmented parallel reduction. */
+void spmm_eb_pr(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_csr_gpu_taco.h:557-587: element-balanced sequential reduction over a
+ * 256-nnz partition (the reference restricts it to N = 128; this one takes
+ * any N). */
+void spmm_csr_gpu_taco(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+
+/* The remaining live names of gpu_kernels.cuh:12-93. Each runs one of the
+ * four schedules with the (group, tile, vector) preset its name encodes;
+ * unlike the reference's tuned kernels, every one is correct for any N. */
+void spmm_csr_gpu_warp(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);      /* spmm_csr_gpu_taco.h:45 */
+void spmm_csr_gpu_reduction(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B); /* spmm_csr_gpu_taco.h:614 */
+void spmm_csr_pr_eb(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);         /* spmm_csr_gpu_taco.h:589 */
+void spmm_rb_sr_taco(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);        /* spmm_rb_sr.h:44 */
+void spmm_rb_pr_taco(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);        /* spmm_rb_pr.h:46 */
+void spmm_eb_sr_taco(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);        /* spmm_eb_sr.h:58 */
+void spmm_eb_pr_taco(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);        /* spmm_eb_pr.h:54 */
+void spmm_eb_sr_search(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);      /* spmm_eb_sr.h:615 */
+void spmm_eb_pr_search(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);      /* spmm_eb_pr.h:1305 */
+void spmm_rb_sr_512_64(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);      /* spmm_rb_sr.h:270 */
+void spmm_eb_sr_512_64(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);      /* spmm_eb_sr.h:763 */
+void spmm_rb_pr_256(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);         /* spmm_rb_pr.h:412 */
+/* spmm_rb_pr.h:469-948: spmm_rb_pr_256_<c>_<g>_<gs> */
+void spmm_rb_pr_256_1_4_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_4_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_4_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_8_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_8_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_8_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_8_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_8_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_8_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_16_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_16_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_16_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_16_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_16_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_16_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_16_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_16_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_16_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_32_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_32_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_32_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_32_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_32_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_32_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_32_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_32_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_32_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_1_32_32(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_2_32_32(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_pr_256_4_32_32(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_eb_pr.h:526-779: spmm_eb_pr_<Nnz>_<gs> */
+void spmm_eb_pr_256_32(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_256_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_256_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_256_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_128_32(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_128_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_128_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_128_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_64_32(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_64_16(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_64_8(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_pr_64_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_rb_sr.h:392-535: spmm_rb_sr_256_<r>_<c> */
+void spmm_rb_sr_256_1_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_1_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_1_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_2_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_2_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_2_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_4_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_4_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_rb_sr_256_4_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+/* spmm_eb_sr.h:908-1095: spmm_eb_sr_256_<g>_<c> */
+void spmm_eb_sr_256_4_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_4_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_4_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_8_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_8_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_8_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_16_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_16_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_eb_sr_256_16_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident API                                                       */
+/* ------------------------------------------------------------------------ */
+#define SEGTACO_DT_F32 0
+#define SEGTACO_DT_BF16 1
+#define SEGTACO_DT_F16 2
+
+#define SEGTACO_RB_SR 0 /* row-balanced, sequential reduction */
+#define SEGTACO_RB_PR 1 /* row-balanced, parallel (tree) reduction */
+#define SEGTACO_EB_SR 2 /* element-balanced, sequential reduction */
+#define SEGTACO_EB_PR 3 /* element-balanced, segmented parallel reduction */
+
+/* A schedule and its parameters. Zero fields are resolved by
+ * segtaco_plan_resolve from the operands. */
+typedef struct segtaco_plan {
+  int32_t schedule;       /* SEGTACO_RB_SR .. SEGTACO_EB_PR */
+  int32_t vec;            /* elements of B per lane per load (1,2,4; 8 for
+                             16-bit storage); 0 = widest N allows */
+  int32_t col_lanes;      /* lanes sharing one B row segment, power of two
+                             1..64; the column tile is col_lanes * vec */
+  int32_t group;          /* *_pr: nnz slots reduced together (power of two);
+                             rb_sr: rows per lane group; 0 = auto */
+  int32_t tile;           /* eb_*: nnz per tile (multiple of 8); 0 = auto */
+  int32_t comp_threshold; /* rows with more nnz than this use compensated
+                             (Neumaier) summation; 0 = never; -1 = default
+                             (1024 for 16-bit storage, never for FP32) */
+} segtaco_plan_t;
+
+/* CSR operand in device memory. vals may be NULL (all ones). */
+typedef struct segtaco_csr {
+  int32_t M, K, nnz;
+  int32_t vals_dtype; /* SEGTACO_DT_*; must equal the B dtype for 16-bit */
+  const int32_t *pos;
+  const int32_t *crd;
+  const void *vals;
+} segtaco_csr_t;
+
+/* Fill zero fields of *plan for these operands; validates the rest. */
+int32_t segtaco_plan_resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
+                             int64_t ldb, int64_t ldc, segtaco_plan_t *plan);
+
+/* Workspace bytes segtaco_spmm needs for this (resolved) plan. */
+size_t segtaco_spmm_workspace_bytes(const segtaco_csr_t *A, int32_t N,
+                                    const segtaco_plan_t *plan);
+
+/* C = A * B on `stream` (a hipStream_t; NULL = default stream). B has dtype
+ * b_dtype, ldb >= N; C is FP32, ldc >= N. workspace may be NULL only when
+ * segtaco_spmm_workspace_bytes is 0. Asynchronous. */
+int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                     int64_t ldb, int32_t N, float *C, int64_t ldc,
+                     const segtaco_plan_t *plan, void *workspace,
+                     size_t workspace_bytes, void *stream);
+
+/* K21 row partition (gpu_library.h:206-215, 236-240), bit-exact:
+ * out[b] = taco_binarySearchBefore(pos, 0, M, b * values_per_block) for
+ * b = 0 .. num_blocks (num_blocks + 1 entries), device pointers. */
+int32_t segtaco_block_starts(const int32_t *pos, int32_t M,
+                             int32_t values_per_block, int32_t num_blocks,
+                             int32_t *out, void *stream);
+
+/* Host-side CSR structure check: pos[0] == 0, pos non-decreasing,
+ * pos[M] == nnz, every crd in [0, K). Host pointers. */
+int32_t segtaco_csr_validate_host(int32_t M, int32_t K, int32_t nnz,
+                                  const int32_t *pos, const int32_t *crd);
+
+/* Thin device-memory helpers for callers without HIP headers. kind:
+ * 0 host->device, 1 device->host, 2 device->device. */
+int32_t segtaco_device_malloc(void **ptr, size_t bytes);
+int32_t segtaco_device_free(void *ptr);
+int32_t segtaco_memcpy(void *dst, const void *src, size_t bytes, int32_t kind);
+int32_t segtaco_memset(void *dst, int32_t value, size_t bytes);
+int32_t segtaco_device_synchronize(void);
+int32_t segtaco_stream_synchronize(void *stream);
+const char *segtaco_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,318 @@
+"""segtaco_amd — Python binding of libsegtaco, the gfx950 CSR SpMM engine.
+
+Everything here is a thin ctypes layer over the C-ABI declared in
+``include/segtaco/*.h``; all compute runs in the HIP library. There is no
+CPU fallback: importing works without a GPU, but every compute call goes
+through libsegtaco.so and raises if it is missing.
+
+Two layers, mirroring the C-ABI:
+
+* :func:`legacy` runs an operator-surface entry point (``spmm_rb_sr``,
+  ``spmm_rb_pr``, ``spmm_eb_sr``, ``spmm_eb_pr``, ``spmm_csr_gpu_taco`` and the
+  other names of the reference's gpu_kernels.cuh) on host numpy operands, with
+  the reference's copy-in / compute / copy-out contract.
+* :func:`spmm` runs the device-resident API on device pointers (integers) with
+  an explicit :class:`Plan`, workspace and stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsegtaco.so")
+
+# Schedules / dtypes / status codes (include/segtaco/spmm.h)
+RB_SR, RB_PR, EB_SR, EB_PR = 0, 1, 2, 3
+SCHEDULES = {"rb_sr": RB_SR, "rb_pr": RB_PR, "eb_sr": EB_SR, "eb_pr": EB_PR}
+DT_F32, DT_BF16, DT_F16 = 0, 1, 2
+DTYPES = {"f32": DT_F32, "bf16": DT_BF16, "fp16": DT_F16}
+OK = 0
+ERRMODE_EXIT, ERRMODE_RECORD = 0, 1
+GEN_UNIFORM, GEN_POWERLAW, GEN_BANDED = 0, 1, 2
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_vp = ctypes.c_void_p
+
+
+class SegtacoError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"segtaco error {status}: {msg}")
+        self.status = status
+
+
+class Plan(ctypes.Structure):
+    """segtaco_plan_t: schedule + (vec, col_lanes, group, tile, comp_threshold).
+    Zero fields are resolved from the operands."""
+    _fields_ = [("schedule", ctypes.c_int32), ("vec", ctypes.c_int32),
+                ("col_lanes", ctypes.c_int32), ("group", ctypes.c_int32),
+                ("tile", ctypes.c_int32), ("comp_threshold", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+    def __repr__(self):
+        return f"Plan({self.as_dict()})"
+
+
+def make_plan(schedule, vec=0, col_lanes=0, group=0, tile=0, comp_threshold=-1) -> Plan:
+    if isinstance(schedule, str):
+        schedule = SCHEDULES[schedule]
+    return Plan(schedule, vec, col_lanes, group, tile, comp_threshold)
+
+
+class CSR(ctypes.Structure):
+    """segtaco_csr_t: device CSR descriptor (pointers are device addresses)."""
+    _fields_ = [("M", ctypes.c_int32), ("K", ctypes.c_int32), ("nnz", ctypes.c_int32),
+                ("vals_dtype", ctypes.c_int32), ("pos", _vp), ("crd", _vp), ("vals", _vp)]
+
+
+class TacoTensor(ctypes.Structure):
+    """taco_tensor_t (taco_tensor_t.h:6-15), identical layout."""
+    _fields_ = [("order", ctypes.c_int32), ("dimensions", _i32p), ("csize", ctypes.c_int32),
+                ("mode_ordering", _i32p), ("mode_types", _vp),
+                ("indices", ctypes.POINTER(ctypes.POINTER(_i32p))), ("vals", _f32p),
+                ("vals_size", ctypes.c_int32)]
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libsegtaco.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libsegtaco.so not built ({LIB_PATH}); run `python -m segtaco_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.segtaco_set_error_mode.restype = ctypes.c_int32
+    L.segtaco_set_error_mode.argtypes = [ctypes.c_int32]
+    L.segtaco_last_error.restype = ctypes.c_int32
+    L.segtaco_last_error_string.restype = ctypes.c_char_p
+    L.segtaco_clear_error.restype = None
+    L.segtaco_version.restype = ctypes.c_char_p
+    L.get_gpu_timer_result.restype = ctypes.c_float
+    L.segtaco_legacy_call.restype = None
+    L.segtaco_legacy_call.argtypes = [ctypes.c_int32, ctypes.POINTER(TacoTensor),
+                                      ctypes.POINTER(TacoTensor), ctypes.POINTER(TacoTensor)]
+    L.segtaco_legacy_count.restype = ctypes.c_int32
+    L.segtaco_legacy_name.restype = ctypes.c_char_p
+    L.segtaco_legacy_name.argtypes = [ctypes.c_int32]
+    L.segtaco_plan_resolve.restype = ctypes.c_int32
+    L.segtaco_plan_resolve.argtypes = [ctypes.POINTER(CSR), ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(Plan)]
+    L.segtaco_spmm_workspace_bytes.restype = ctypes.c_size_t
+    L.segtaco_spmm_workspace_bytes.argtypes = [ctypes.POINTER(CSR), ctypes.c_int32,
+                                               ctypes.POINTER(Plan)]
+    L.segtaco_spmm.restype = ctypes.c_int32
+    L.segtaco_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32, ctypes.c_int64,
+                               ctypes.c_int32, _vp, ctypes.c_int64, ctypes.POINTER(Plan), _vp,
+                               ctypes.c_size_t, _vp]
+    L.segtaco_block_starts.restype = ctypes.c_int32
+    L.segtaco_block_starts.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       _vp, _vp]
+    L.segtaco_csr_validate_host.restype = ctypes.c_int32
+    L.segtaco_csr_validate_host.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            _i32p, _i32p]
+    L.segtaco_device_malloc.restype = ctypes.c_int32
+    L.segtaco_device_malloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
+    L.segtaco_device_free.restype = ctypes.c_int32
+    L.segtaco_device_free.argtypes = [_vp]
+    L.segtaco_memcpy.restype = ctypes.c_int32
+    L.segtaco_memcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int32]
+    L.segtaco_device_synchronize.restype = ctypes.c_int32
+    L.segtaco_stream_synchronize.restype = ctypes.c_int32
+    L.segtaco_stream_synchronize.argtypes = [_vp]
+    L.segtaco_gen_pos.restype = ctypes.c_int64
+    L.segtaco_gen_pos.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int32, ctypes.c_int32, _i32p]
+    L.segtaco_gen_fill.restype = ctypes.c_int32
+    L.segtaco_gen_fill.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f32p]
+    L.segtaco_gen_dense.restype = ctypes.c_int32
+    L.segtaco_gen_dense.argtypes = [ctypes.c_uint64, ctypes.c_int64, _f32p]
+    L.segtaco_convert_f32.restype = ctypes.c_int32
+    L.segtaco_convert_f32.argtypes = [ctypes.c_int32, ctypes.c_int64, _f32p, _u16p]
+    _LIB = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != OK:
+        L = lib()
+        msg = (L.segtaco_last_error_string() or b"").decode()
+        raise SegtacoError(status, msg)
+
+
+def version() -> str:
+    return lib().segtaco_version().decode()
+
+
+# ---------------------------------------------------------------------------
+# Synthetic operands (host, numpy)
+# ---------------------------------------------------------------------------
+
+def gen_csr(kind: int, seed: int, M: int, K: int, mean: float = 16.0, sigma: float = 1.5,
+            cap: int = 8192, bw: int = 32, values: bool = True):
+    """Seeded synthetic CSR: (pos int32[M+1], crd int32[nnz], vals float32[nnz])."""
+    L = lib()
+    pos = np.empty(M + 1, dtype=np.int32)
+    nnz = L.segtaco_gen_pos(kind, seed, M, K, mean, sigma, cap, bw,
+                            pos.ctypes.data_as(_i32p))
+    if nnz < 0:
+        raise SegtacoError(1, "segtaco_gen_pos failed (bad arguments or nnz > INT32_MAX)")
+    crd = np.empty(max(nnz, 1), dtype=np.int32)
+    vals = np.empty(max(nnz, 1), dtype=np.float32) if values else None
+    check(L.segtaco_gen_fill(kind, seed, M, K, bw, pos.ctypes.data_as(_i32p),
+                             crd.ctypes.data_as(_i32p),
+                             vals.ctypes.data_as(_f32p) if values else None))
+    return pos, crd[:nnz], (vals[:nnz] if values else None)
+
+
+def gen_dense(seed: int, rows: int, cols: int) -> np.ndarray:
+    out = np.empty((rows, cols), dtype=np.float32)
+    check(lib().segtaco_gen_dense(seed, rows * cols, out.ctypes.data_as(_f32p)))
+    return out
+
+
+def to_half(dtype: str | int, x: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16/FP16 bit patterns (uint16), round to nearest even."""
+    code = DTYPES[dtype] if isinstance(dtype, str) else dtype
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, dtype=np.uint16)
+    check(lib().segtaco_convert_f32(code, x.size, x.ctypes.data_as(_f32p),
+                                    out.ctypes.data_as(_u16p)))
+    return out
+
+
+def validate_csr(M: int, K: int, pos: np.ndarray, crd: np.ndarray) -> None:
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    crd = np.ascontiguousarray(crd, dtype=np.int32)
+    nnz = int(pos[M]) if M > 0 else 0
+    crd_p = crd.ctypes.data_as(_i32p) if crd.size else None
+    check(lib().segtaco_csr_validate_host(M, K, nnz, pos.ctypes.data_as(_i32p), crd_p))
+
+
+# ---------------------------------------------------------------------------
+# Operator surface (host operands)
+# ---------------------------------------------------------------------------
+
+def legacy_names() -> list[str]:
+    L = lib()
+    return [L.segtaco_legacy_name(i).decode() for i in range(L.segtaco_legacy_count())]
+
+
+def _legacy_id(name: str) -> int:
+    names = legacy_names()
+    if name not in names:
+        raise KeyError(f"unknown operator {name}")
+    return names.index(name)
+
+
+@dataclass
+class _Keep:
+    """Keeps the ctypes buffers of a taco_tensor_t alive."""
+    objs: list
+
+
+def _taco_dense(arr: np.ndarray, keep: list) -> TacoTensor:
+    dims = (ctypes.c_int32 * 2)(arr.shape[0], arr.shape[1])
+    order = (ctypes.c_int32 * 2)(0, 1)
+    keep += [dims, order, arr]
+    t = TacoTensor()
+    t.order = 2
+    t.dimensions = ctypes.cast(dims, _i32p)
+    t.mode_ordering = ctypes.cast(order, _i32p)
+    t.vals = arr.ctypes.data_as(_f32p)
+    t.vals_size = arr.size
+    return t
+
+
+def _taco_csr(M: int, K: int, pos, crd, vals, keep: list) -> TacoTensor:
+    dims = (ctypes.c_int32 * 2)(M, K)
+    idx1 = (_i32p * 2)(pos.ctypes.data_as(_i32p), crd.ctypes.data_as(_i32p))
+    idx = (ctypes.POINTER(_i32p) * 2)(None, ctypes.cast(idx1, ctypes.POINTER(_i32p)))
+    keep += [dims, idx1, idx, pos, crd, vals]
+    t = TacoTensor()
+    t.order = 2
+    t.dimensions = ctypes.cast(dims, _i32p)
+    t.indices = ctypes.cast(idx, ctypes.POINTER(ctypes.POINTER(_i32p)))
+    t.vals = vals.ctypes.data_as(_f32p) if vals is not None else None
+    t.vals_size = int(pos[M]) if M > 0 else 0
+    return t
+
+
+def legacy(name: str, pos, crd, vals, B: np.ndarray, K: int | None = None,
+           C: np.ndarray | None = None):
+    """Run operator-surface entry point `name` on host operands.
+
+    Returns (C float32 [M, N], elapsed ms from get_gpu_timer_result()).
+    Errors raise SegtacoError (the library runs in record mode here)."""
+    L = lib()
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    crd = np.ascontiguousarray(crd, dtype=np.int32)
+    if crd.size == 0:
+        crd = np.zeros(1, dtype=np.int32)
+    if vals is not None:
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    M = len(pos) - 1
+    K = B.shape[0] if K is None else K
+    N = B.shape[1]
+    if C is None:
+        C = np.full((M, N), np.nan, dtype=np.float32)
+    keep: list = []
+    tA = _taco_csr(M, K, pos, crd, vals, keep)
+    tB = _taco_dense(B, keep)
+    tC = _taco_dense(C, keep)
+    prev = L.segtaco_set_error_mode(ERRMODE_RECORD)
+    try:
+        L.segtaco_clear_error()
+        L.segtaco_legacy_call(_legacy_id(name), ctypes.byref(tC), ctypes.byref(tA),
+                              ctypes.byref(tB))
+        check(L.segtaco_last_error())
+    finally:
+        L.segtaco_set_error_mode(prev)
+    return C, float(L.get_gpu_timer_result())
+
+
+# ---------------------------------------------------------------------------
+# Device-resident API (device pointers as integers)
+# ---------------------------------------------------------------------------
+
+def csr_desc(M: int, K: int, nnz: int, pos_ptr: int, crd_ptr: int, vals_ptr: int | None,
+             vals_dtype: int = DT_F32) -> CSR:
+    return CSR(M, K, nnz, vals_dtype, pos_ptr, crd_ptr, vals_ptr or None)
+
+
+def plan_resolve(A: CSR, N: int, b_dtype: int, plan: Plan, ldb: int | None = None,
+                 ldc: int | None = None) -> Plan:
+    p = Plan(*[getattr(plan, f) for f, _ in Plan._fields_])
+    check(lib().segtaco_plan_resolve(ctypes.byref(A), N, b_dtype, ldb or N, ldc or N,
+                                     ctypes.byref(p)))
+    return p
+
+
+def workspace_bytes(A: CSR, N: int, plan: Plan) -> int:
+    return int(lib().segtaco_spmm_workspace_bytes(ctypes.byref(A), N, ctypes.byref(plan)))
+
+
+def spmm(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: int,
+         plan: Plan, ws_ptr: int | None, ws_bytes: int, stream: int | None = None) -> None:
+    """C = A * B on device memory (asynchronous on `stream`)."""
+    check(lib().segtaco_spmm(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
+                             ctypes.byref(plan), ws_ptr or None, ws_bytes, stream or None))
+
+
+def block_starts(pos_ptr: int, M: int, T: int, num_blocks: int, out_ptr: int,
+                 stream: int | None = None) -> None:
+    """K21 partition on device (gpu_library.h:206-215)."""
+    check(lib().segtaco_block_starts(pos_ptr, M, T, num_blocks, out_ptr, stream or None))

@@ -1,0 +1,358 @@
+// engine.cpp — the device-resident SpMM API: plan resolution, workspace
+// layout, validation and kernel sequencing for the four schedules.
+#include "engine.h"
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+
+#include "launch.h"
+
+namespace segtaco {
+
+namespace {
+thread_local int32_t t_last_status = SEGTACO_OK;
+thread_local char t_last_msg[512] = "";
+}  // namespace
+
+int32_t set_error(int32_t status, const char *fmt, ...) {
+  t_last_status = status;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_last_msg, sizeof(t_last_msg), fmt, ap);
+  va_end(ap);
+  return status;
+}
+
+int32_t hip_error(hipError_t e, const char *what) {
+  const int32_t st =
+      e == hipErrorOutOfMemory ? SEGTACO_ERR_NOMEM : SEGTACO_ERR_HIP;
+  return set_error(st, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile) {
+  EbLayout L{};
+  L.ntiles = (int32_t)ceil_div(nnz, tile);
+  const size_t part = sizeof(int32_t) * ((size_t)L.ntiles + 1);
+  const size_t carry = sizeof(float) * (size_t)L.ntiles * (size_t)N;
+  L.part_off = 0;
+  L.head_off = align_up(part, 256);
+  L.tail_off = L.head_off + align_up(carry, 256);
+  L.total = L.tail_off + align_up(carry, 256);
+  return L;
+}
+
+namespace {
+
+bool aligned(const void *p, size_t a) {
+  return ((uintptr_t)p % a) == 0;
+}
+
+// Widest per-lane vector (in elements) that N, the leading dimensions and the
+// base pointers allow.
+int32_t widest_vec(int32_t dt, int32_t N, int64_t ldb, int64_t ldc,
+                   const void *B, const void *C) {
+  const int32_t maxv = dt == SEGTACO_DT_F32 ? 4 : 8;
+  for (int32_t v = maxv; v > 1; v >>= 1) {
+    if (N % v || ldb % v || ldc % v) continue;
+    if (B && !aligned(B, (size_t)v * elem_bytes(dt))) continue;
+    if (C && !aligned(C, (size_t)v * 4)) continue;
+    return v;
+  }
+  return 1;
+}
+
+int32_t natural_lanes(int32_t N, int32_t vec) {
+  int64_t q = pow2_ceil(ceil_div(N, vec));
+  return (int32_t)(q > 64 ? 64 : q);
+}
+
+int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
+                int64_t ldb, int64_t ldc, const void *B, const void *C,
+                segtaco_plan_t *plan) {
+  if (!A || !plan) return set_error(SEGTACO_ERR_INVALID, "null argument");
+  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_EB_PR)
+    return set_error(SEGTACO_ERR_INVALID, "unknown schedule %d",
+                     plan->schedule);
+  if (b_dtype < SEGTACO_DT_F32 || b_dtype > SEGTACO_DT_F16)
+    return set_error(SEGTACO_ERR_INVALID, "unknown B dtype %d", b_dtype);
+  if (A->vals && A->vals_dtype != b_dtype)
+    return set_error(SEGTACO_ERR_UNSUPPORTED,
+                     "vals dtype %d differs from B dtype %d", A->vals_dtype,
+                     b_dtype);
+  if (N < 1) N = 1;
+  const int32_t maxv = b_dtype == SEGTACO_DT_F32 ? 4 : 8;
+  if (plan->vec == 0) {
+    plan->vec = widest_vec(b_dtype, N, ldb, ldc, B, C);
+  } else {
+    const int32_t v = plan->vec;
+    if (!is_pow2(v) || v > maxv)
+      return set_error(SEGTACO_ERR_INVALID, "vec %d not in 1..%d", v, maxv);
+    if (N % v || ldb % v || ldc % v ||
+        (B && !aligned(B, (size_t)v * elem_bytes(b_dtype))) ||
+        (C && !aligned(C, (size_t)v * 4)))
+      return set_error(SEGTACO_ERR_UNSUPPORTED,
+                       "vec %d does not divide N/ld or align the operands", v);
+  }
+  const int32_t V = plan->vec;
+  const int32_t qn = natural_lanes(N, V);
+  switch (plan->schedule) {
+    case SEGTACO_RB_SR:
+      if (plan->col_lanes == 0) plan->col_lanes = qn;
+      if (plan->group == 0) plan->group = 1;
+      break;
+    case SEGTACO_RB_PR:
+      if (plan->col_lanes == 0) {
+        int32_t g = plan->group ? plan->group : 1;
+        int32_t q = qn;
+        while (q * g > 64 && q > 1) q >>= 1;
+        plan->col_lanes = q;
+      }
+      if (plan->group == 0) plan->group = 64 / plan->col_lanes;
+      break;
+    case SEGTACO_EB_SR:
+      if (plan->col_lanes == 0) plan->col_lanes = qn;
+      if (plan->tile == 0) plan->tile = 128;
+      if (plan->group == 0) plan->group = 1;
+      break;
+    case SEGTACO_EB_PR:
+      if (plan->col_lanes == 0 && plan->group != 0)
+        plan->col_lanes = 64 / plan->group;
+      if (plan->col_lanes == 0) plan->col_lanes = qn < 8 ? qn : 8;
+      if (plan->group == 0) plan->group = 64 / plan->col_lanes;
+      if (plan->tile == 0) plan->tile = 256;
+      break;
+  }
+  const int32_t Q = plan->col_lanes, G = plan->group;
+  if (!is_pow2(Q) || Q > 64)
+    return set_error(SEGTACO_ERR_INVALID, "col_lanes %d not a power of two <= 64", Q);
+  if (!is_pow2(G) || G > 64)
+    return set_error(SEGTACO_ERR_INVALID, "group %d not a power of two <= 64", G);
+  if (plan->schedule == SEGTACO_RB_PR && Q * G > 64)
+    return set_error(SEGTACO_ERR_INVALID, "rb_pr: col_lanes*group = %d > 64", Q * G);
+  if (plan->schedule == SEGTACO_EB_PR && Q * G != 64)
+    return set_error(SEGTACO_ERR_INVALID, "eb_pr: col_lanes*group = %d != 64", Q * G);
+  if (plan->schedule == SEGTACO_RB_SR && G > 16)
+    return set_error(SEGTACO_ERR_INVALID, "rb_sr: rows per group %d > 16", G);
+  if (plan->schedule >= SEGTACO_EB_SR &&
+      (plan->tile < 8 || plan->tile > (1 << 22)))
+    return set_error(SEGTACO_ERR_INVALID, "tile %d not in [8, 2^22]", plan->tile);
+  if (plan->comp_threshold == -1)
+    plan->comp_threshold = b_dtype == SEGTACO_DT_F32 ? 0 : 1024;
+  if (plan->comp_threshold < 0)
+    return set_error(SEGTACO_ERR_INVALID, "comp_threshold %d", plan->comp_threshold);
+  return SEGTACO_OK;
+}
+
+int32_t check_csr(const segtaco_csr_t *A) {
+  if (!A) return set_error(SEGTACO_ERR_INVALID, "null CSR");
+  if (A->M < 0 || A->K < 0 || A->nnz < 0)
+    return set_error(SEGTACO_ERR_INVALID, "negative CSR size");
+  if (A->M > 0 && !A->pos) return set_error(SEGTACO_ERR_INVALID, "null pos");
+  if (A->nnz > 0 && !A->crd) return set_error(SEGTACO_ERR_INVALID, "null crd");
+  return SEGTACO_OK;
+}
+
+}  // namespace
+
+size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
+                                 const segtaco_plan_t *plan) {
+  if (!A || !plan || A->nnz == 0 || A->M == 0) return 0;
+  if (plan->schedule != SEGTACO_EB_SR && plan->schedule != SEGTACO_EB_PR)
+    return 0;
+  const int32_t tile = plan->tile ? plan->tile : 128;
+  return eb_layout(A->nnz, N, tile).total;
+}
+
+int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                  int64_t ldb, int32_t N, float *C, int64_t ldc,
+                  const segtaco_plan_t *plan_in, void *ws, size_t ws_bytes,
+                  hipStream_t stream) {
+  SEGTACO_RET(check_csr(A));
+  if (!plan_in) return set_error(SEGTACO_ERR_INVALID, "null plan");
+  if (N < 0) return set_error(SEGTACO_ERR_INVALID, "N < 0");
+  if (A->M == 0 || N == 0) return SEGTACO_OK;
+  if (!C) return set_error(SEGTACO_ERR_INVALID, "null C");
+  if (ldc < N) return set_error(SEGTACO_ERR_INVALID, "ldc < N");
+  if (A->nnz == 0) {
+    SEGTACO_HIP(launch_fill_zero(C, A->M, N, ldc, stream));
+    return SEGTACO_OK;
+  }
+  if (!B) return set_error(SEGTACO_ERR_INVALID, "null B");
+  if (ldb < N) return set_error(SEGTACO_ERR_INVALID, "ldb < N");
+  segtaco_plan_t plan = *plan_in;
+  SEGTACO_RET(resolve(A, N, b_dtype, ldb, ldc, B, C, &plan));
+
+  KArgs a{};
+  a.M = A->M;
+  a.K = A->K;
+  a.N = N;
+  a.nnz = A->nnz;
+  a.pos = A->pos;
+  a.crd = A->crd;
+  a.vals = A->vals;
+  a.B = B;
+  a.ldb = ldb;
+  a.C = C;
+  a.ldc = ldc;
+  a.lq = ilog2(plan.col_lanes);
+  a.lg = ilog2(plan.group);
+  a.tile = plan.tile;
+  a.comp_threshold = plan.comp_threshold == 0 ? INT_MAX : plan.comp_threshold;
+  const int32_t V = plan.vec;
+  a.panels = (int32_t)ceil_div(N, (int64_t)plan.col_lanes * V);
+
+  switch (plan.schedule) {
+    case SEGTACO_RB_SR: {
+      const int64_t rows_per_block =
+          (int64_t)kWavesPerBlock * (64 / plan.col_lanes) * plan.group;
+      a.nblocks = (int32_t)ceil_div(A->M, rows_per_block);
+      SEGTACO_HIP(launch_rb_sr(b_dtype, V, a, stream));
+      return SEGTACO_OK;
+    }
+    case SEGTACO_RB_PR: {
+      const int64_t rows_per_wave = 64 / (plan.col_lanes * plan.group);
+      a.nblocks = (int32_t)ceil_div(A->M, rows_per_wave * kWavesPerBlock);
+      SEGTACO_HIP(launch_rb_pr(b_dtype, V, a, stream));
+      return SEGTACO_OK;
+    }
+    default:
+      break;
+  }
+  // Element-balanced: partition, tiles, carry fix-up.
+  const EbLayout L = eb_layout(A->nnz, N, plan.tile);
+  if (!ws || ws_bytes < L.total)
+    return set_error(SEGTACO_ERR_INVALID,
+                     "workspace of %zu bytes needed, %zu given", L.total,
+                     ws ? ws_bytes : (size_t)0);
+  char *w = static_cast<char *>(ws);
+  a.ntiles = L.ntiles;
+  a.rows_per_slice = (int32_t)ceil_div(A->M, L.ntiles);
+  a.part = reinterpret_cast<int32_t *>(w + L.part_off);
+  a.carry_head = reinterpret_cast<float *>(w + L.head_off);
+  a.carry_tail = reinterpret_cast<float *>(w + L.tail_off);
+  SEGTACO_HIP(launch_block_starts(A->pos, A->M, plan.tile, L.ntiles,
+                                  const_cast<int32_t *>(a.part), stream));
+  if (plan.schedule == SEGTACO_EB_SR) {
+    const int64_t tiles_per_block =
+        (int64_t)kWavesPerBlock * (64 / plan.col_lanes);
+    a.nblocks = (int32_t)ceil_div(L.ntiles, tiles_per_block);
+    SEGTACO_HIP(launch_eb_sr(b_dtype, V, a, stream));
+  } else {
+    a.nblocks = (int32_t)ceil_div(L.ntiles, kWavesPerBlock);
+    SEGTACO_HIP(launch_eb_pr(b_dtype, V, a, stream));
+  }
+  if (L.ntiles > 1) SEGTACO_HIP(launch_fixup(V, a, stream));
+  return SEGTACO_OK;
+}
+
+int32_t plan_resolve_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
+                          int64_t ldb, int64_t ldc, segtaco_plan_t *plan) {
+  return resolve(A, N, b_dtype, ldb, ldc, nullptr, nullptr, plan);
+}
+
+}  // namespace segtaco
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace segtaco;
+
+extern "C" {
+
+int32_t segtaco_last_error(void) { return t_last_status; }
+const char *segtaco_last_error_string(void) { return t_last_msg; }
+void segtaco_clear_error(void) {
+  t_last_status = SEGTACO_OK;
+  t_last_msg[0] = '\0';
+}
+
+const char *segtaco_version(void) { return "segtaco-mi355x 0.1 (gfx950)"; }
+
+int32_t segtaco_plan_resolve(const segtaco_csr_t *A, int32_t N,
+                             int32_t b_dtype, int64_t ldb, int64_t ldc,
+                             segtaco_plan_t *plan) {
+  return plan_resolve_impl(A, N, b_dtype, ldb, ldc, plan);
+}
+
+size_t segtaco_spmm_workspace_bytes(const segtaco_csr_t *A, int32_t N,
+                                    const segtaco_plan_t *plan) {
+  return spmm_workspace_bytes_impl(A, N, plan);
+}
+
+int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                     int64_t ldb, int32_t N, float *C, int64_t ldc,
+                     const segtaco_plan_t *plan, void *workspace,
+                     size_t workspace_bytes, void *stream) {
+  return spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace,
+                   workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int32_t segtaco_block_starts(const int32_t *pos, int32_t M,
+                             int32_t values_per_block, int32_t num_blocks,
+                             int32_t *out, void *stream) {
+  if (!pos || !out || M < 0 || values_per_block < 1 || num_blocks < 0)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_block_starts: bad argument");
+  SEGTACO_HIP(launch_block_starts(pos, M, values_per_block, num_blocks, out,
+                                  static_cast<hipStream_t>(stream)));
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_csr_validate_host(int32_t M, int32_t K, int32_t nnz,
+                                  const int32_t *pos, const int32_t *crd) {
+  if (M < 0 || K < 0 || nnz < 0)
+    return set_error(SEGTACO_ERR_INVALID, "negative CSR size");
+  if (!pos) return set_error(SEGTACO_ERR_INVALID, "null pos");
+  if (pos[0] != 0) return set_error(SEGTACO_ERR_INVALID, "pos[0] = %d", pos[0]);
+  for (int32_t r = 0; r < M; ++r)
+    if (pos[r + 1] < pos[r])
+      return set_error(SEGTACO_ERR_INVALID, "pos decreases at row %d", r);
+  if (pos[M] != nnz)
+    return set_error(SEGTACO_ERR_INVALID, "pos[M] = %d != nnz = %d", pos[M], nnz);
+  if (nnz > 0 && !crd) return set_error(SEGTACO_ERR_INVALID, "null crd");
+  for (int32_t p = 0; p < nnz; ++p)
+    if (crd[p] < 0 || crd[p] >= K)
+      return set_error(SEGTACO_ERR_INVALID, "crd[%d] = %d outside [0, %d)", p,
+                       crd[p], K);
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_device_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return set_error(SEGTACO_ERR_INVALID, "null ptr");
+  *ptr = nullptr;
+  if (bytes == 0) return SEGTACO_OK;
+  SEGTACO_HIP(hipMalloc(ptr, bytes));
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_device_free(void *ptr) {
+  if (ptr) SEGTACO_HIP(hipFree(ptr));
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_memcpy(void *dst, const void *src, size_t bytes, int32_t kind) {
+  if (bytes == 0) return SEGTACO_OK;
+  hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost
+                                : hipMemcpyDeviceToDevice;
+  SEGTACO_HIP(hipMemcpy(dst, src, bytes, k));
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_memset(void *dst, int32_t value, size_t bytes) {
+  if (bytes == 0) return SEGTACO_OK;
+  SEGTACO_HIP(hipMemset(dst, value, bytes));
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_device_synchronize(void) {
+  SEGTACO_HIP(hipDeviceSynchronize());
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_stream_synchronize(void *stream) {
+  SEGTACO_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return SEGTACO_OK;
+}
+
+}  // extern "C"

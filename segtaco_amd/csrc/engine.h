@@ -1,0 +1,53 @@
+// engine.h — internal host-side helpers shared by the libsegtaco sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "segtaco/spmm.h"
+
+namespace segtaco {
+
+// Records (status, message) as this thread's last error and returns status.
+int32_t set_error(int32_t status, const char *fmt, ...);
+int32_t hip_error(hipError_t e, const char *what);
+
+#define SEGTACO_HIP(call)                                     \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::segtaco::hip_error(e_, #call); \
+  } while (0)
+
+#define SEGTACO_RET(call)                 \
+  do {                                    \
+    int32_t s_ = (call);                  \
+    if (s_ != SEGTACO_OK) return s_;      \
+  } while (0)
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+inline int32_t ilog2(int64_t x) {
+  int32_t l = 0;
+  while ((int64_t(1) << (l + 1)) <= x) ++l;
+  return l;
+}
+inline int64_t pow2_ceil(int64_t x) {
+  int64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+inline int32_t elem_bytes(int32_t dt) { return dt == SEGTACO_DT_F32 ? 4 : 2; }
+
+// Workspace layout of an element-balanced plan.
+struct EbLayout {
+  int32_t ntiles;
+  size_t part_off, head_off, tail_off, total;
+};
+EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile);
+
+}  // namespace segtaco

@@ -1,0 +1,152 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol the public
+headers declare (both linkages of the operator surface), and its host-side
+logic (plan resolution, workspace sizing, CSR validation, generators) behaves.
+No compute call is made (no GPU here)."""
+import glob
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import segtaco_amd as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "segtaco", "*.h")))
+
+
+def _declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text,
+                             flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", S.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_library_loads():
+    assert "gfx950" in S.version()
+
+
+def test_every_declared_symbol_is_exported():
+    declared = _declared_functions()
+    assert len(declared) > 90
+    missing = declared - _exported()
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+
+
+def test_operator_surface_matches_reference_names():
+    # The live names of the reference's gpu_kernels.cuh:11-93 minus the two
+    # cuSPARSE comparators (spmm_csr_gpu_cusparse, spmm_csr_cusparse_row).
+    names = S.legacy_names()
+    assert len(names) == 77
+    for must in ("spmm_rb_sr", "spmm_rb_pr", "spmm_eb_sr", "spmm_eb_pr", "spmm_csr_gpu_taco"):
+        assert must in names
+    assert set(names) <= _declared_functions()
+
+
+def test_cxx_linkage_aliases_exported():
+    exported = _exported()
+    for name in S.legacy_names():
+        mangled = f"_Z{len(name)}{name}P13taco_tensor_tS0_S0_"
+        assert mangled in exported, mangled
+    assert "_Z20get_gpu_timer_resultv" in exported
+
+
+def _desc(M=100, K=80, nnz=1000, dtype=S.DT_F32):
+    return S.csr_desc(M, K, nnz, 0x1000, 0x2000, 0x3000, dtype)
+
+
+def test_plan_resolution_defaults():
+    A = _desc()
+    p = S.plan_resolve(A, 128, S.DT_F32, S.make_plan("rb_sr"))
+    assert (p.vec, p.col_lanes, p.group) == (4, 32, 1)
+    assert p.comp_threshold == 0  # FP32: no compensation by default
+    p = S.plan_resolve(A, 32, S.DT_F32, S.make_plan("rb_pr"))
+    assert p.vec == 4 and p.col_lanes * p.group <= 64
+    p = S.plan_resolve(_desc(dtype=S.DT_BF16), 256, S.DT_BF16, S.make_plan("eb_sr"), 256, 256)
+    assert p.vec == 8 and p.col_lanes == 32 and p.tile > 0 and p.comp_threshold == 1024
+    p = S.plan_resolve(A, 128, S.DT_F32, S.make_plan("eb_pr", group=16))
+    assert p.col_lanes * p.group == 64
+    p = S.plan_resolve(A, 7, S.DT_F32, S.make_plan("rb_sr"))
+    assert p.vec == 1 and p.col_lanes == 8
+    p = S.plan_resolve(_desc(dtype=S.DT_F16), 6, S.DT_F16, S.make_plan("rb_sr"), 6, 6)
+    assert p.vec == 2
+
+
+@pytest.mark.parametrize("bad", [
+    dict(schedule=7), dict(schedule=0, vec=3), dict(schedule=0, col_lanes=3),
+    dict(schedule=1, col_lanes=16, group=8), dict(schedule=3, col_lanes=8, group=4),
+    dict(schedule=2, tile=4), dict(schedule=0, vec=8)])
+def test_plan_resolution_rejects(bad):
+    with pytest.raises(S.SegtacoError):
+        S.plan_resolve(_desc(), 128, S.DT_F32, S.make_plan(**bad))
+
+
+def test_vec_must_divide_n():
+    with pytest.raises(S.SegtacoError):
+        S.plan_resolve(_desc(), 6, S.DT_F32, S.make_plan("rb_sr", vec=4))
+
+
+def test_workspace_bytes():
+    A = _desc(nnz=1000)
+    p = S.plan_resolve(A, 64, S.DT_F32, S.make_plan("eb_sr", tile=100))
+    ntiles = 10
+    need = S.workspace_bytes(A, 64, p)
+    assert need >= 4 * (ntiles + 1) + 2 * 4 * ntiles * 64
+    assert S.workspace_bytes(A, 64, S.plan_resolve(A, 64, S.DT_F32, S.make_plan("rb_sr"))) == 0
+
+
+def test_validate_csr():
+    pos = np.array([0, 2, 2, 5], np.int32)
+    crd = np.array([0, 3, 1, 2, 4], np.int32)
+    S.validate_csr(3, 5, pos, crd)
+    with pytest.raises(S.SegtacoError, match="outside"):
+        S.validate_csr(3, 4, pos, crd)
+    with pytest.raises(S.SegtacoError, match="decreases"):
+        S.validate_csr(3, 5, np.array([0, 2, 1, 5], np.int32), crd)
+    with pytest.raises(S.SegtacoError, match=r"pos\[0\]"):
+        S.validate_csr(3, 5, np.array([1, 2, 2, 5], np.int32), crd)
+
+
+def test_legacy_rejects_malformed_operands_before_gpu_work():
+    pos = np.array([0, 2, 3], np.int32)
+    crd = np.array([0, 9, 1], np.int32)  # 9 >= K
+    B = np.ones((4, 3), np.float32)
+    with pytest.raises(S.SegtacoError, match="outside"):
+        S.legacy("spmm_rb_sr", pos, crd, np.ones(3, np.float32), B)
+    with pytest.raises(S.SegtacoError, match="columns"):
+        S.legacy("spmm_eb_sr", pos, np.array([0, 1, 2], np.int32), None, B, K=4,
+                 C=np.zeros((2, 5), np.float32))
+
+
+def test_generators_deterministic_and_shaped():
+    a = S.gen_csr(S.GEN_POWERLAW, 9, 5000, 5000)
+    b = S.gen_csr(S.GEN_POWERLAW, 9, 5000, 5000)
+    c = S.gen_csr(S.GEN_POWERLAW, 10, 5000, 5000)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert not np.array_equal(a[1], c[1])
+    pos, crd, vals = a
+    S.validate_csr(5000, 5000, pos, crd)
+    L = np.diff(pos)
+    assert 12 < L.mean() < 20 and (L == 0).any()
+    for i in range(0, 5000, 97):
+        row = crd[pos[i]:pos[i + 1]]
+        assert np.all(np.diff(row) > 0)
+    assert vals.min() >= -1 and vals.max() <= 1
+    pb, cb, _ = S.gen_csr(S.GEN_BANDED, 1, 500, 500, bw=32)
+    assert int(pb[-1]) == 65 * 500 - 2 * sum(range(33))
+    pu, _, _ = S.gen_csr(S.GEN_UNIFORM, 1, 64, 4096, mean=16)
+    assert np.all(np.diff(pu) == 16)
+    d1 = S.gen_dense(3, 100, 7)
+    assert np.array_equal(d1, S.gen_dense(3, 100, 7)) and abs(d1.mean()) < 0.1

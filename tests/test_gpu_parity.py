@@ -1,0 +1,206 @@
+"""GPU parity of the four schedules against the CPU oracle, through the C-ABI.
+
+Tolerances (stated on the metric the oracle defines, oracle.scaled_err:
+max |C - ref| / sum_p |v_p| |B[crd_p, k]|):
+  * FP32 storage: <= 1e-5 against the FP64 closed form and against the
+    Eigen-order FP32 product (the reference's CPU path).
+  * Partition arrays: bit-exact.
+  * Every schedule is atomic-free: two runs are bit-identical.
+"""
+import glob
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import segtaco_amd as S
+from gpu_util import DeviceCSR, dense_dev, dev, run_spmm, stream_ptr, torch
+
+pytestmark = pytest.mark.gpu
+
+TOL_F32 = 1e-5
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+PLANS = [
+    ("rb_sr", {}), ("rb_sr", dict(group=4)), ("rb_sr", dict(vec=1)),
+    ("rb_pr", {}), ("rb_pr", dict(group=8)), ("rb_pr", dict(col_lanes=4, group=4)),
+    ("rb_pr", dict(col_lanes=1, group=32)),
+    ("eb_sr", {}), ("eb_sr", dict(tile=8)), ("eb_sr", dict(tile=33)), ("eb_sr", dict(tile=1000)),
+    ("eb_pr", {}), ("eb_pr", dict(group=32, tile=64)), ("eb_pr", dict(group=4, tile=512)),
+    ("eb_pr", dict(group=64, tile=8)),
+]
+PLAN_IDS = [f"{s}-" + "-".join(f"{k}{v}" for k, v in kw.items()) for s, kw in PLANS]
+
+
+def _check(pos, crd, vals, B, C, tol=TOL_F32, what=""):
+    ref = O.spmm_f64(pos, crd, vals, B)
+    scale = O.spmm_absbound(pos, crd, vals, B)
+    err = O.scaled_err(C, ref, scale)
+    assert err <= tol, f"{what}: scaled error {err:.3e} > {tol:.1e}"
+    eig = O.spmm_f32_seq(pos, crd, vals, B)
+    err2 = O.scaled_err(C, eig.astype(np.float64), scale)
+    assert err2 <= tol, f"{what}: vs Eigen order {err2:.3e}"
+    return err
+
+
+def _device_run(pos, crd, vals, B, sched, kw, K=None, runs=1):
+    K = B.shape[0] if K is None else K
+    A = DeviceCSR(pos, crd, vals, K)
+    N = B.shape[1]
+    Bd = dense_dev(B)
+    outs = []
+    for _ in range(runs):
+        C = run_spmm(A, Bd, N, S.make_plan(sched, **kw))
+        outs.append(C[: A.M].cpu().numpy())
+    return outs
+
+
+# ---------------------------------------------------------------------------
+# Operand families
+# ---------------------------------------------------------------------------
+
+def _case(name):
+    if name.startswith("golden:"):
+        d = np.load(name.split(":", 1)[1], allow_pickle=False)
+        vals = d["vals"] if int(d["has_vals"]) else None
+        return d["pos"], d["crd"], vals, d["B"]
+    if name == "cfg1_uniform_4096_n32":
+        pos, crd, vals = S.gen_csr(S.GEN_UNIFORM, 1, 4096, 4096, mean=16)
+        return pos, crd, vals, S.gen_dense(11, 4096, 32)
+    if name == "powerlaw_20000_n128":
+        pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 2, 20000, 20000)
+        return pos, crd, vals, S.gen_dense(12, 20000, 128)
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    if name == "empty_runs_n20":
+        M, K = 3000, 500
+        lens = rng.integers(1, 12, M)
+        for s in rng.integers(0, M - 60, 25):
+            lens[s: s + rng.integers(1, 60)] = 0
+        lens[:17] = 0
+        lens[-23:] = 0
+        return _mk(rng, lens, K, 20)
+    if name == "one_long_row_n130":
+        lens = np.array([0, 3, 20000, 0, 5, 9000, 1], np.int64)
+        return _mk(rng, lens, 25000, 130)
+    if name == "spmv_n1":
+        lens = rng.integers(0, 40, 5000)
+        return _mk(rng, lens, 4000, 1)
+    if name == "n3_odd":
+        lens = rng.integers(0, 30, 2000)
+        return _mk(rng, lens, 2000, 3)
+    if name == "n512_panels":
+        lens = rng.integers(0, 24, 1500)
+        return _mk(rng, lens, 1500, 512)
+    if name == "single_row":
+        return _mk(rng, np.array([777]), 1000, 64)
+    if name == "all_empty_n8":
+        return _mk(rng, np.zeros(100, np.int64), 50, 8)
+    if name == "novals_n16":
+        pos, crd, _, B = _mk(rng, rng.integers(0, 50, 800), 900, 16)
+        return pos, crd, None, B
+    raise KeyError(name)
+
+
+def _mk(rng, lens, K, N):
+    lens = np.minimum(lens, K)
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    crd = np.concatenate([np.sort(rng.choice(K, int(l), replace=False)) for l in lens]
+                         + [np.zeros(0, np.int64)]).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(crd)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    return pos, crd, vals, B
+
+
+CASES = ([f"golden:{p}" for p in GOLDEN] +
+         ["cfg1_uniform_4096_n32", "powerlaw_20000_n128", "empty_runs_n20", "one_long_row_n130",
+          "spmv_n1", "n3_odd", "n512_panels", "single_row", "all_empty_n8", "novals_n16"])
+CASE_IDS = [c.split("/")[-1] for c in CASES]
+
+
+@pytest.mark.parametrize("case", CASES, ids=CASE_IDS)
+@pytest.mark.parametrize("sched,kw", PLANS, ids=PLAN_IDS)
+def test_schedule_parity(case, sched, kw):
+    pos, crd, vals, B = _case(case)
+    print(f"[parity] {case} {sched} {kw} M={len(pos)-1} N={B.shape[1]}", flush=True)
+    C, C2 = _device_run(pos, crd, vals, B, sched, kw, runs=2)
+    _check(pos, crd, vals, B, C, what=f"{case}/{sched}{kw}")
+    assert np.array_equal(C, C2), "not deterministic"
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_golden_against_eigen_output(path):
+    d = np.load(path, allow_pickle=False)
+    vals = d["vals"] if int(d["has_vals"]) else None
+    scale = O.spmm_absbound(d["pos"], d["crd"], vals, d["B"])
+    for sched in ("rb_sr", "rb_pr", "eb_sr", "eb_pr"):
+        (C,) = _device_run(d["pos"], d["crd"], vals, d["B"], sched, {})
+        err = O.scaled_err(C, d["C_eigen"].astype(np.float64), scale)
+        assert err <= TOL_F32, f"{sched}: {err}"
+
+
+def test_strided_operands():
+    """ldb > N and ldc > N (row-major with padding)."""
+    t = torch()
+    pos, crd, vals, B = _case("powerlaw_20000_n128")
+    N = 40
+    Bp = np.zeros((B.shape[0], 48), np.float32)
+    Bp[:, :N] = B[:, :N]
+    A = DeviceCSR(pos, crd, vals, B.shape[0])
+    for sched in ("rb_sr", "rb_pr", "eb_sr", "eb_pr"):
+        C = t.full((A.M, 44), 7.0, dtype=t.float32, device="cuda")
+        run_spmm(A, dev(Bp), N, S.make_plan(sched), ldb=48, ldc=44, C=C)
+        Ch = C.cpu().numpy()
+        _check(pos, crd, vals, B[:, :N].copy(), Ch[:, :N].copy(), what=sched)
+        assert np.all(Ch[:, N:] == 7.0), "wrote outside [0, N)"
+
+
+@pytest.mark.parametrize("T", [1, 7, 32, 256, 1024])
+def test_partition_bitexact(T):
+    rng = np.random.default_rng(T)
+    lens = rng.integers(0, 9, 50000)
+    lens[rng.random(50000) < 0.3] = 0
+    lens[1000:1800] = 0
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    M, nnz = len(pos) - 1, int(pos[-1])
+    nb = (nnz + T - 1) // T
+    out = torch().full((nb + 1,), -1, dtype=torch().int32, device="cuda")
+    S.block_starts(dev(pos).data_ptr(), M, T, nb, out.data_ptr(), stream_ptr())
+    torch().cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), O.block_starts(pos, M, T, nb))
+
+
+def test_full_size_powerlaw_all_schedules():
+    """The power-law workload at full size (262144^2, N = 128)."""
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 2, 262144, 262144)
+    B = S.gen_dense(3, 262144, 128)
+    ref = O.spmm_f64(pos, crd, vals, B)
+    scale = O.spmm_absbound(pos, crd, vals, B)
+    A = DeviceCSR(pos, crd, vals, 262144)
+    Bd = dense_dev(B)
+    for sched in ("rb_sr", "rb_pr", "eb_sr", "eb_pr"):
+        print(f"[full] {sched}", flush=True)
+        C = run_spmm(A, Bd, 128, S.make_plan(sched)).cpu().numpy()
+        err = O.scaled_err(C, ref, scale)
+        assert err <= TOL_F32, f"{sched}: {err}"
+
+
+@pytest.mark.parametrize("name", [
+    "spmm_rb_sr", "spmm_rb_pr", "spmm_eb_sr", "spmm_eb_pr", "spmm_csr_gpu_taco"])
+def test_north_star_entry_points(name):
+    pos, crd, vals, B = _case("empty_runs_n20")
+    C, ms = S.legacy(name, pos, crd, vals, B)
+    _check(pos, crd, vals, B, C, what=name)
+    assert ms > 0
+
+
+def test_all_operator_surface_names():
+    pos, crd, vals, B = _case("n3_odd")
+    pos2, crd2, vals2, B2 = _case("cfg1_uniform_4096_n32")
+    for name in S.legacy_names():
+        print(f"[legacy] {name}", flush=True)
+        C, _ = S.legacy(name, pos, crd, vals, B)
+        _check(pos, crd, vals, B, C, what=name)
+        C2, _ = S.legacy(name, pos2, crd2, vals2, B2)
+        _check(pos2, crd2, vals2, B2, C2, what=name)
