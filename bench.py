@@ -94,7 +94,13 @@ def main():
     ap.add_argument("--steps", type=int, default=25)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--schedules", default="rb_sr,rb_pr,eb_sr,eb_pr")
+    ap.add_argument("--plans", default="",
+                    help="';'-separated plans to time instead of --schedules, each "
+                         "'schedule[:key=value[,key=value]]', e.g. 'eb_sr:tile=256;rb_pr:group=4'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run two plain 128 MiB device copies (known bytes) so that "
+                         "rocprofv3 --pmc byte counters can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this SpMM "
@@ -122,12 +128,27 @@ def main():
     A = S.csr_desc(M, K, nnz, d_pos.data_ptr(), d_crd.data_ptr(), d_val.data_ptr())
     stream = torch.cuda.current_stream()
 
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 2 x 134217728-byte device copies")
+
     flops = 2.0 * nnz * N
     cbytes = compulsory_bytes(M, k_touched, N, nnz)
     results = {}
-    for name in args.schedules.split(","):
-        rp, ms, per = time_plan(torch, A, Bd, C, N, S.make_plan(name), args.steps, args.warmup,
-                                stream)
+    plans = []
+    if args.plans:
+        for spec in args.plans.split(";"):
+            name, _, kv = spec.partition(":")
+            kw = {k: int(v) for k, v in (x.split("=") for x in kv.split(",") if x)}
+            plans.append((spec, S.make_plan(name, **kw)))
+    else:
+        plans = [(name, S.make_plan(name)) for name in args.schedules.split(",")]
+    for name, plan in plans:
+        rp, ms, per = time_plan(torch, A, Bd, C, N, plan, args.steps, args.warmup, stream)
         results[name] = {"ms": ms, "ms_min": min(per), "ms_median": statistics.median(per),
                          "gflops": flops / (ms * 1e-3) / 1e9,
                          "hbm_frac": cbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

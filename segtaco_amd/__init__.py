@@ -23,7 +23,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsegtaco.so")
+# SEGTACO_LIB points at another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("SEGTACO_LIB") or os.path.join(_HERE, "libsegtaco.so")
 
 # Schedules / dtypes / status codes (include/segtaco/spmm.h)
 RB_SR, RB_PR, EB_SR, EB_PR = 0, 1, 2, 3

@@ -87,6 +87,77 @@ __device__ __forceinline__ void load_vec(const void *__restrict__ base,
   }
 }
 
+// Packed (not yet widened) V-element slice of a B row: what a gather leaves
+// in registers while in flight. unpack() widens it to FP32 at use.
+template <int DT, int V>
+struct RawT;
+template <>
+struct RawT<0, 4> { using type = float4; };
+template <>
+struct RawT<0, 2> { using type = float2; };
+template <>
+struct RawT<0, 1> { using type = float; };
+template <int DT>
+struct RawT<DT, 8> { using type = uint4; };
+template <int DT>
+struct RawT<DT, 4> { using type = uint2; };
+template <int DT>
+struct RawT<DT, 2> { using type = uint32_t; };
+template <int DT>
+struct RawT<DT, 1> { using type = uint16_t; };
+
+template <int DT, int V>
+__device__ __forceinline__ typename RawT<DT, V>::type load_raw(
+    const void *__restrict__ base, int64_t i) {
+  using R = typename RawT<DT, V>::type;
+  if constexpr (DT == 0) {
+    return *reinterpret_cast<const R *>(static_cast<const float *>(base) + i);
+  } else {
+    return *reinterpret_cast<const R *>(static_cast<const uint16_t *>(base) + i);
+  }
+}
+
+template <int DT, int V>
+__device__ __forceinline__ void unpack(const typename RawT<DT, V>::type &x,
+                                       float (&out)[V]) {
+  if constexpr (DT == 0) {
+    if constexpr (V == 4) {
+      out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = x.w;
+    } else if constexpr (V == 2) {
+      out[0] = x.x; out[1] = x.y;
+    } else {
+      out[0] = x;
+    }
+  } else {
+    if constexpr (V == 8) {
+      const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        out[2 * k] = half_to_f32<DT>(w[k]);
+        out[2 * k + 1] = half_to_f32<DT>(w[k] >> 16);
+      }
+    } else if constexpr (V == 4) {
+      out[0] = half_to_f32<DT>(x.x);
+      out[1] = half_to_f32<DT>(x.x >> 16);
+      out[2] = half_to_f32<DT>(x.y);
+      out[3] = half_to_f32<DT>(x.y >> 16);
+    } else if constexpr (V == 2) {
+      out[0] = half_to_f32<DT>(x);
+      out[1] = half_to_f32<DT>(x >> 16);
+    } else {
+      out[0] = half_to_f32<DT>(x);
+    }
+  }
+}
+
+// Makes this wave's LDS writes visible to all of its lanes: a release/acquire
+// fence at wavefront scope around a wave barrier (no block-wide s_barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // V consecutive FP32 values to C (aligned to V floats; V == 8 is two float4).
 template <int V>
 __device__ __forceinline__ void store_vec(float *__restrict__ p,

@@ -36,7 +36,8 @@ EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile) {
   const size_t part = sizeof(int32_t) * ((size_t)L.ntiles + 1);
   const size_t carry = sizeof(float) * (size_t)L.ntiles * (size_t)N;
   L.part_off = 0;
-  L.head_off = align_up(part, 256);
+  L.rows_off = align_up(part, 256);
+  L.head_off = L.rows_off + align_up(part, 256);
   L.tail_off = L.head_off + align_up(carry, 256);
   L.total = L.tail_off + align_up(carry, 256);
   return L;
@@ -231,8 +232,10 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   a.part = reinterpret_cast<int32_t *>(w + L.part_off);
   a.carry_head = reinterpret_cast<float *>(w + L.head_off);
   a.carry_tail = reinterpret_cast<float *>(w + L.tail_off);
+  a.tile_row = reinterpret_cast<int32_t *>(w + L.rows_off);
   SEGTACO_HIP(launch_block_starts(A->pos, A->M, plan.tile, L.ntiles,
-                                  const_cast<int32_t *>(a.part), stream));
+                                  const_cast<int32_t *>(a.part),
+                                  const_cast<int32_t *>(a.tile_row), stream));
   if (plan.schedule == SEGTACO_EB_SR) {
     const int64_t tiles_per_block =
         (int64_t)kWavesPerBlock * (64 / plan.col_lanes);
@@ -242,7 +245,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     a.nblocks = (int32_t)ceil_div(L.ntiles, kWavesPerBlock);
     SEGTACO_HIP(launch_eb_pr(b_dtype, V, a, stream));
   }
-  if (L.ntiles > 1) SEGTACO_HIP(launch_fixup(V, a, stream));
+  SEGTACO_HIP(launch_fixup(V, a, stream));
   return SEGTACO_OK;
 }
 
@@ -294,7 +297,7 @@ int32_t segtaco_block_starts(const int32_t *pos, int32_t M,
   if (!pos || !out || M < 0 || values_per_block < 1 || num_blocks < 0)
     return set_error(SEGTACO_ERR_INVALID, "segtaco_block_starts: bad argument");
   SEGTACO_HIP(launch_block_starts(pos, M, values_per_block, num_blocks, out,
-                                  static_cast<hipStream_t>(stream)));
+                                  nullptr, static_cast<hipStream_t>(stream)));
   return SEGTACO_OK;
 }
 

@@ -46,7 +46,7 @@ inline int32_t elem_bytes(int32_t dt) { return dt == SEGTACO_DT_F32 ? 4 : 2; }
 // Workspace layout of an element-balanced plan.
 struct EbLayout {
   int32_t ntiles;
-  size_t part_off, head_off, tail_off, total;
+  size_t part_off, rows_off, head_off, tail_off, total;
 };
 EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile);
 

@@ -28,6 +28,7 @@ struct KArgs {
   int32_t nblocks;         // row / tile blocks before panel expansion
   int32_t rows_per_slice;  // eb_*: rows per tile for empty-row zero fill
   const int32_t *part;     // eb_*: K21 partition, ntiles + 1 entries
+  const int32_t *tile_row; // eb_*: row holding each tile's first entry
   float *carry_head;       // eb_*: [ntiles][N] partial of a continued row
   float *carry_tail;       // eb_*: [ntiles][N] partial of a row left open
 };
@@ -38,9 +39,14 @@ hipError_t launch_eb_sr(int dt, int vec, const KArgs &a, hipStream_t s);
 hipError_t launch_eb_pr(int dt, int vec, const KArgs &a, hipStream_t s);
 // Combines carry partials of rows that span eb tiles (vec in floats).
 hipError_t launch_fixup(int vec, const KArgs &a, hipStream_t s);
+// out[b] = K21 search for b*T (bit-exact); rows (optional) gets the row that
+// holds entry b*T after the reference's repair step (M when b*T >= nnz).
 hipError_t launch_block_starts(const int32_t *pos, int32_t M, int32_t T,
-                               int32_t num_blocks, int32_t *out,
+                               int32_t num_blocks, int32_t *out, int32_t *rows,
                                hipStream_t s);
+// Dynamic LDS bytes a launch of k_eb_sr / k_eb_pr uses.
+size_t eb_sr_lds_bytes(const KArgs &a);
+size_t eb_pr_lds_bytes(const KArgs &a);
 hipError_t launch_fill_zero(float *C, int32_t M, int32_t N, int64_t ldc,
                             hipStream_t s);
 
