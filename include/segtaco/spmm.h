@@ -237,6 +237,9 @@ int32_t segtaco_memset(void *dst, int32_t value, size_t bytes);
 int32_t segtaco_device_synchronize(void);
 int32_t segtaco_stream_synchronize(void *stream);
 const char *segtaco_version(void);
+/* GPUs visible to the HIP runtime libsegtaco is bound to; a negative value is
+ * minus the HIP error code of hipGetDeviceCount. */
+int32_t segtaco_device_count(void);
 
 #ifdef __cplusplus
 }

@@ -197,6 +197,19 @@ def scaled_err(C: np.ndarray, ref: np.ndarray, scale: np.ndarray) -> float:
                                          _ptr(scale, _f64p)))
 
 
+def rel_norm_errs(C: np.ndarray, ref: np.ndarray):
+    """(||C - ref||_F / ||ref||_F, max|C - ref| / max|ref|): norm-wise error
+    measures for storage-precision checks, where an element whose magnitude is
+    below the storage format's subnormal range may legitimately lose all of
+    its own digits."""
+    C = np.asarray(C, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    d = C - ref
+    fro = float(np.linalg.norm(d) / max(np.linalg.norm(ref), 1e-300))
+    mx = float(np.abs(d).max() / max(np.abs(ref).max(), 1e-300)) if ref.size else 0.0
+    return fro, mx
+
+
 def round_to(dtype: str, x: np.ndarray):
     """Round float32 -> ('bf16'|'fp16') with RNE; returns (uint16 bits, widened float32)."""
     code = {"bf16": 1, "fp16": 2}[dtype]

@@ -73,6 +73,25 @@ class CSR(ctypes.Structure):
                 ("vals_dtype", ctypes.c_int32), ("pos", _vp), ("crd", _vp), ("vals", _vp)]
 
 
+HIST_BINS = 24
+
+
+class RowStats(ctypes.Structure):
+    """segtaco_row_stats_t: row-length statistics the tuner reads."""
+    _fields_ = [("M", ctypes.c_int32), ("K", ctypes.c_int32), ("nnz", ctypes.c_int32),
+                ("max_len", ctypes.c_int32), ("empty_rows", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("mean_len", ctypes.c_double),
+                ("cv", ctypes.c_double), ("frac_nnz_long", ctypes.c_double),
+                ("hist", ctypes.c_int64 * HIST_BINS)]
+
+
+class BatchItem(ctypes.Structure):
+    """segtaco_batch_item_t."""
+    _fields_ = [("A", CSR), ("B", _vp), ("ldb", ctypes.c_int64), ("C", _vp),
+                ("ldc", ctypes.c_int64), ("N", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("plan", Plan), ("pos_host", _i32p)]
+
+
 class TacoTensor(ctypes.Structure):
     """taco_tensor_t (taco_tensor_t.h:6-15), identical layout."""
     _fields_ = [("order", ctypes.c_int32), ("dimensions", _i32p), ("csize", ctypes.c_int32),
@@ -85,7 +104,12 @@ _LIB = None
 
 
 def lib() -> ctypes.CDLL:
-    """Load libsegtaco.so (raises if it was not built)."""
+    """Load libsegtaco.so (raises if it was not built).
+
+    Mixing with PyTorch in one process: torch bundles its own HIP/HSA runtime
+    and libsegtaco binds to whichever libamdhip64.so.7 is loaded first, so
+    import torch before the first call into this module; loading libsegtaco
+    first and torch afterwards leaves two HSA runtimes in the process."""
     global _LIB
     if _LIB is not None:
         return _LIB
@@ -99,6 +123,7 @@ def lib() -> ctypes.CDLL:
     L.segtaco_last_error_string.restype = ctypes.c_char_p
     L.segtaco_clear_error.restype = None
     L.segtaco_version.restype = ctypes.c_char_p
+    L.segtaco_device_count.restype = ctypes.c_int32
     L.get_gpu_timer_result.restype = ctypes.c_float
     L.segtaco_legacy_call.restype = None
     L.segtaco_legacy_call.argtypes = [ctypes.c_int32, ctypes.POINTER(TacoTensor),
@@ -142,6 +167,48 @@ def lib() -> ctypes.CDLL:
     L.segtaco_gen_dense.argtypes = [ctypes.c_uint64, ctypes.c_int64, _f32p]
     L.segtaco_convert_f32.restype = ctypes.c_int32
     L.segtaco_convert_f32.argtypes = [ctypes.c_int32, ctypes.c_int64, _f32p, _u16p]
+    L.segtaco_row_stats.restype = ctypes.c_int32
+    L.segtaco_row_stats.argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.POINTER(RowStats)]
+    L.segtaco_tune_model.restype = ctypes.c_int32
+    L.segtaco_tune_model.argtypes = [ctypes.POINTER(RowStats), ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.POINTER(Plan)]
+    L.segtaco_tune.restype = ctypes.c_int32
+    L.segtaco_tune.argtypes = [ctypes.POINTER(RowStats), ctypes.c_int32, ctypes.c_int32,
+                               ctypes.POINTER(Plan)]
+    L.segtaco_tune_measure.restype = ctypes.c_int32
+    L.segtaco_tune_measure.argtypes = [ctypes.POINTER(CSR), ctypes.POINTER(RowStats), _vp,
+                                       ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _vp,
+                                       ctypes.c_int64, ctypes.c_int32, _vp, ctypes.POINTER(Plan),
+                                       ctypes.POINTER(ctypes.c_float)]
+    L.segtaco_tune_signature.restype = ctypes.c_uint64
+    L.segtaco_tune_signature.argtypes = [ctypes.POINTER(RowStats), ctypes.c_int32,
+                                         ctypes.c_int32]
+    L.segtaco_tune_cache_get.restype = ctypes.c_int32
+    L.segtaco_tune_cache_get.argtypes = [ctypes.c_uint64, ctypes.POINTER(Plan)]
+    L.segtaco_tune_cache_put.restype = ctypes.c_int32
+    L.segtaco_tune_cache_put.argtypes = [ctypes.c_uint64, ctypes.POINTER(Plan)]
+    L.segtaco_tune_cache_clear.restype = ctypes.c_int32
+    L.segtaco_tune_cache_save.restype = ctypes.c_int32
+    L.segtaco_tune_cache_save.argtypes = [ctypes.c_char_p]
+    L.segtaco_tune_cache_load.restype = ctypes.c_int32
+    L.segtaco_tune_cache_load.argtypes = [ctypes.c_char_p]
+    L.segtaco_batch_create.restype = ctypes.c_int32
+    L.segtaco_batch_create.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_int32,
+                                       ctypes.c_int32, _vp, ctypes.POINTER(_vp)]
+    for fn in ("segtaco_batch_run", "segtaco_batch_replay"):
+        getattr(L, fn).restype = ctypes.c_int32
+        getattr(L, fn).argtypes = [_vp, _vp]
+    L.segtaco_batch_capture.restype = ctypes.c_int32
+    L.segtaco_batch_capture.argtypes = [_vp]
+    L.segtaco_batch_plan.restype = ctypes.c_int32
+    L.segtaco_batch_plan.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(Plan)]
+    L.segtaco_batch_units.restype = ctypes.c_int64
+    L.segtaco_batch_units.argtypes = [_vp]
+    L.segtaco_batch_workspace_bytes.restype = ctypes.c_size_t
+    L.segtaco_batch_workspace_bytes.argtypes = [_vp]
+    L.segtaco_batch_destroy.restype = ctypes.c_int32
+    L.segtaco_batch_destroy.argtypes = [_vp]
     _LIB = L
     return L
 
@@ -155,6 +222,11 @@ def check(status: int) -> None:
 
 def version() -> str:
     return lib().segtaco_version().decode()
+
+
+def device_count() -> int:
+    """GPUs visible to libsegtaco's HIP runtime (negative HIP error code on failure)."""
+    return int(lib().segtaco_device_count())
 
 
 # ---------------------------------------------------------------------------
@@ -317,3 +389,135 @@ def block_starts(pos_ptr: int, M: int, T: int, num_blocks: int, out_ptr: int,
                  stream: int | None = None) -> None:
     """K21 partition on device (gpu_library.h:206-215)."""
     check(lib().segtaco_block_starts(pos_ptr, M, T, num_blocks, out_ptr, stream or None))
+
+
+# ---------------------------------------------------------------------------
+# Autotuner (include/segtaco/autotune.h)
+# ---------------------------------------------------------------------------
+
+def row_stats(pos: np.ndarray, K: int) -> RowStats:
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    st = RowStats()
+    check(lib().segtaco_row_stats(pos.ctypes.data_as(_i32p), len(pos) - 1, K, ctypes.byref(st)))
+    return st
+
+
+def tune_model(st: RowStats, N: int, dtype: str | int = "f32") -> Plan:
+    dt = DTYPES[dtype] if isinstance(dtype, str) else dtype
+    p = Plan()
+    check(lib().segtaco_tune_model(ctypes.byref(st), N, dt, ctypes.byref(p)))
+    return p
+
+
+def tune(st: RowStats, N: int, dtype: str | int = "f32") -> Plan:
+    """Cached measurement for the statistics' signature, else the model."""
+    dt = DTYPES[dtype] if isinstance(dtype, str) else dtype
+    p = Plan()
+    check(lib().segtaco_tune(ctypes.byref(st), N, dt, ctypes.byref(p)))
+    return p
+
+
+def tune_signature(st: RowStats, N: int, dtype: str | int = "f32") -> int:
+    dt = DTYPES[dtype] if isinstance(dtype, str) else dtype
+    return int(lib().segtaco_tune_signature(ctypes.byref(st), N, dt))
+
+
+def tune_cache_get(sig: int):
+    p = Plan()
+    return p if lib().segtaco_tune_cache_get(sig, ctypes.byref(p)) == OK else None
+
+
+def tune_cache_put(sig: int, plan: Plan) -> None:
+    check(lib().segtaco_tune_cache_put(sig, ctypes.byref(plan)))
+
+
+def tune_cache_clear() -> None:
+    check(lib().segtaco_tune_cache_clear())
+
+
+def tune_cache_save(path: str) -> None:
+    check(lib().segtaco_tune_cache_save(path.encode()))
+
+
+def tune_cache_load(path: str) -> None:
+    check(lib().segtaco_tune_cache_load(path.encode()))
+
+
+def tune_measure(A: CSR, st: RowStats, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int,
+                 ldc: int, reps: int = 3, stream: int | None = None):
+    """Times candidate plans on the device; returns (best plan, its ms)."""
+    p = Plan()
+    ms = ctypes.c_float()
+    check(lib().segtaco_tune_measure(ctypes.byref(A), ctypes.byref(st), B_ptr, b_dtype, ldb, N,
+                                     C_ptr, ldc, reps, stream or None, ctypes.byref(p),
+                                     ctypes.byref(ms)))
+    return p, float(ms.value)
+
+
+# ---------------------------------------------------------------------------
+# Batched persistent launch (include/segtaco/batch.h)
+# ---------------------------------------------------------------------------
+
+SCHEDULE_AUTO = -1
+
+
+def batch_item(A: CSR, B_ptr: int, ldb: int, C_ptr: int, ldc: int, N: int,
+               plan: Plan | None = None, pos_host: np.ndarray | None = None) -> BatchItem:
+    it = BatchItem()
+    it.A = A
+    it.B = B_ptr
+    it.ldb = ldb
+    it.C = C_ptr
+    it.ldc = ldc
+    it.N = N
+    it.plan = plan if plan is not None else make_plan(0, comp_threshold=-1)
+    if plan is None:
+        it.plan.schedule = SCHEDULE_AUTO
+    if pos_host is not None:
+        it.pos_host = pos_host.ctypes.data_as(_i32p)
+    return it
+
+
+class Batch:
+    """Owns a segtaco_batch_t: create once, run / capture / replay many times."""
+
+    def __init__(self, items: list, dtype: str | int = "f32", stream: int | None = None):
+        self.dtype = DTYPES[dtype] if isinstance(dtype, str) else dtype
+        arr = (BatchItem * max(len(items), 1))(*items)
+        self._items = arr
+        h = _vp()
+        check(lib().segtaco_batch_create(arr, len(items), self.dtype, stream or None,
+                                         ctypes.byref(h)))
+        self.handle = h
+        self.n = len(items)
+
+    def run(self, stream: int | None = None) -> None:
+        check(lib().segtaco_batch_run(self.handle, stream or None))
+
+    def capture(self) -> None:
+        check(lib().segtaco_batch_capture(self.handle))
+
+    def replay(self, stream: int | None = None) -> None:
+        check(lib().segtaco_batch_replay(self.handle, stream or None))
+
+    def plan(self, i: int) -> Plan:
+        p = Plan()
+        check(lib().segtaco_batch_plan(self.handle, i, ctypes.byref(p)))
+        return p
+
+    def units(self) -> int:
+        return int(lib().segtaco_batch_units(self.handle))
+
+    def workspace_bytes(self) -> int:
+        return int(lib().segtaco_batch_workspace_bytes(self.handle))
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().segtaco_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

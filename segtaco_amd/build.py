@@ -33,7 +33,8 @@ COMMON = [
 ]
 
 LIB_SOURCES = [
-    "k_rb.hip", "k_eb.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp", "gen.cpp",
+    "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
+    "gen.cpp", "autotune.cpp", "batch.cpp",
 ]
 
 

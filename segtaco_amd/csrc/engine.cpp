@@ -272,6 +272,16 @@ void segtaco_clear_error(void) {
 
 const char *segtaco_version(void) { return "segtaco-mi355x 0.1 (gfx950)"; }
 
+int32_t segtaco_device_count(void) {
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    hip_error(e, "hipGetDeviceCount");
+    return -(int32_t)e;
+  }
+  return n;
+}
+
 int32_t segtaco_plan_resolve(const segtaco_csr_t *A, int32_t N,
                              int32_t b_dtype, int64_t ldb, int64_t ldc,
                              segtaco_plan_t *plan) {

@@ -1,225 +1,18 @@
-// k_eb.hip — element-balanced schedules for gfx950 (wave64), the K21 row
-// partition, and the carry fix-up that replaces the reference's atomics.
-//
-// Stored entries are cut into tiles of T consecutive entries (tile t covers
-// [t*T, min((t+1)*T, nnz))). The partition pass (k_block_starts) computes the
-// reference's K21 array (gpu_library.h:206-215) bit-exactly and, from the same
-// search, each tile's first row after the reference's repair step that skips
-// empty rows (e.g. spmm_eb_sr.h:39-42).
-//
-// Carry-out without atomics (the reference uses atomicAdd for every row that
-// crosses a warp or block, spmm_eb_sr.h:232-256, spmm_eb_pr.h:340-371):
-//   * a row that starts and ends inside tile t is written to C by tile t;
-//   * a row that started before tile t contributes its partial to
-//     carry_head[t]; a row that starts in tile t and continues past it
-//     contributes to carry_tail[t];
-//   * k_fixup writes every row that spans tiles t0..t1 as
-//     carry_tail[t0] + carry_head[t0+1] + ... + carry_head[t1], in tile order,
-//     and writes the empty rows of tile t's row slice
-//     [t*rows_per_slice, (t+1)*rows_per_slice) as zeros.
-// Every C element has exactly one writer and a fixed summation order: results
-// are bit-identical run to run, and C needs no clearing.
-//
-// Both schedules stage each chunk of their tile in LDS with coalesced loads —
-// column indices, values and a window of the row pointer starting at the
-// chunk's first row — so that the walk over the chunk issues only B-row
-// gathers to memory (the reference re-reads pos per row and, in K3/K5,
-// bisects all of pos per entry, spmm_eb_sr.h:215, spmm_eb_pr.h:320).
-//
-// eb_sr (reference K3/K4/K14/K18: spmm_eb_sr.h:5-78, 153-712,
-// spmm_csr_gpu_taco.h:114-163): a lane group of Q lanes walks its tile in
-// entry order with two batches of gathers in flight, flushing each row as
-// it ends.
-// eb_pr (reference K5/K6/K16/K17: spmm_eb_pr.h:11-75, 281-946): one wave per
-// tile; P = 64/Q slots take P consecutive entries per pass; a segmented
-// suffix scan over the slots (lane distances Q, 2Q, .. 32) leaves each row
-// segment's sum in its first slot; a segment that runs into the next pass is
-// carried in registers.
-#include <climits>
-
-#include "device_common.h"
+// k_eb.hip — standalone launches of the element-balanced schedules and their
+// fix-up (bodies in sched_eb.h), the K21 row partition and a zero fill.
+#include "sched_eb.h"
 
 namespace segtaco {
 
-constexpr int32_t kPosSlack = 8;   // extra pos entries staged past a chunk
-constexpr int32_t kSrChunkMax = 128;
-#ifndef SEGTACO_SR_BATCH
-#define SEGTACO_SR_BATCH 8
-#endif
-// eb_sr gathers per batch (two batches in flight per lane group)
-constexpr int kSrBatch = SEGTACO_SR_BATCH;
-constexpr int32_t kPrChunk = 128;
-constexpr int kPrPasses = 8;       // eb_pr passes whose gathers are in flight
-
-// eb_sr chunk: the LDS per block stays under 64 KiB for every lane count.
-__host__ __device__ inline int32_t eb_sr_chunk(int32_t lq) {
-  const int32_t c = 16 << lq;
-  return c < kSrChunkMax ? c : kSrChunkMax;
-}
-
-size_t eb_sr_lds_bytes(const KArgs &a) {
-  const int32_t ch = eb_sr_chunk(a.lq);
-  const size_t groups = (size_t)kWavesPerBlock << (6 - a.lq);
-  return groups * (size_t)(3 * ch + kPosSlack) * sizeof(int32_t);
-}
-
-size_t eb_pr_lds_bytes(const KArgs &) {
-  return (size_t)kWavesPerBlock * (4 * kPrChunk + kPosSlack) * sizeof(int32_t);
-}
-
-// Zero the empty rows of [r0, r1) for column panel `panel` (all Q*V columns
-// of the panel), each row checked by one of `nl` cooperating lanes.
-template <int V>
-__device__ __forceinline__ void zero_empty_rows(const KArgs &a, int32_t r0,
-                                                int32_t r1, int32_t lane_in,
-                                                int32_t nl, int32_t panel) {
-  const int32_t *__restrict__ pos = a.pos;
-  const int32_t Q = 1 << a.lq;
-  float z[V];
-  zero_vec(z);
-  for (int32_t r = r0 + lane_in; r < r1; r += nl) {
-    if (pos[r] == pos[r + 1]) {
-      float *crow = a.C + (int64_t)r * a.ldc;
-      for (int32_t qq = 0; qq < Q; ++qq) {
-        const int32_t cc = ((panel << a.lq) + qq) * V;
-        if (cc < a.N) store_vec<V>(crow + cc, z);
-      }
-    }
-  }
-}
+size_t eb_sr_lds_bytes(const KArgs &a) { return eb_sr_lds_bytes_lq(a.lq); }
+size_t eb_pr_lds_bytes(const KArgs &) { return eb_pr_lds_bytes_fixed(); }
 
 template <int DT, int V>
 __global__ __launch_bounds__(kBlockThreads) void k_eb_sr(const KArgs a) {
   extern __shared__ int32_t smem[];
   int32_t blk, panel;
   if (!decode_block(a.panels, a.nblocks, blk, panel)) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int lq = a.lq;
-  const int32_t Q = 1 << lq;
-  const int32_t q = lane & (Q - 1);
-  const int32_t gib = (wave << (6 - lq)) + (lane >> lq);  // group in block
-  const int32_t t = blk * (kWavesPerBlock << (6 - lq)) + gib;
-  const int32_t CH = eb_sr_chunk(lq);
-  const int32_t PW = CH + kPosSlack;
-  int32_t *s_crd = smem + gib * (3 * CH + kPosSlack);
-  float *s_val = reinterpret_cast<float *>(s_crd + CH);
-  int32_t *s_pos = s_crd + 2 * CH;
-
-  const int32_t *__restrict__ pos = a.pos;
-  const int32_t *__restrict__ crd = a.crd;
-  const void *__restrict__ vals = a.vals;
-  const void *__restrict__ B = a.B;
-  const int64_t ldb = a.ldb;
-  const int32_t col = ((panel << lq) + q) * V;
-  const bool colok = col < a.N;
-  const bool tok = t < a.ntiles;
-  const int32_t s = tok ? t * a.tile : 0;
-  const int32_t e =
-      tok ? (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile) : 0;
-
-  int32_t r = tok ? a.tile_row[t] : 0;
-  int32_t rs = 0, re = 0;
-  bool comp = false;
-  float acc[V], cmp[V];
-  zero_vec(acc);
-  zero_vec(cmp);
-  int32_t rbase = 0, pw = 0;
-
-  auto P = [&](int32_t row) -> int32_t {
-    const int32_t i = row - rbase;
-    return i < pw ? s_pos[i] : pos[row];
-  };
-  auto flush = [&]() {
-    if (!colok) return;
-    const bool before = rs < s;
-    const bool after = re > e;
-    if (comp) {
-#pragma unroll
-      for (int j = 0; j < V; ++j) acc[j] += cmp[j];
-    }
-    float *dst = before  ? a.carry_head + (int64_t)t * a.N + col
-                 : after ? a.carry_tail + (int64_t)t * a.N + col
-                         : a.C + (int64_t)r * a.ldc + col;
-    store_vec<V>(dst, acc);
-  };
-  auto next_row = [&](int32_t pp) {
-    flush();
-    ++r;
-    while (P(r + 1) <= pp) ++r;  // skip empty rows (zeroed by k_fixup)
-    rs = P(r);
-    re = P(r + 1);
-    comp = (re - rs) > a.comp_threshold;
-    zero_vec(acc);
-    zero_vec(cmp);
-  };
-
-  using R = typename RawT<DT, V>::type;
-  const int32_t nchunks = (a.tile + CH - 1) / CH;  // uniform over the block
-  for (int32_t c = 0; c < nchunks; ++c) {
-    const int32_t cs = s + c * CH;
-    const int32_t cn = max(0, min(e - cs, CH));
-    if (cn > 0) {
-      for (int32_t i = q; i < cn; i += Q) {
-        s_crd[i] = crd[cs + i];
-        s_val[i] = load_val<DT>(vals, cs + i);
-      }
-      rbase = r;
-      pw = min(PW, a.M + 1 - rbase);
-      for (int32_t i = q; i < pw; i += Q) s_pos[i] = pos[rbase + i];
-    }
-    wave_sync();
-    if (cn > 0) {
-      if (c == 0) {
-        rs = P(r);
-        re = P(r + 1);
-        comp = (re - rs) > a.comp_threshold;
-      }
-      // Double-buffered: batch k+1's gathers are in flight while batch k is
-      // consumed, so the memory pipe never drains between batches.
-      R ra[kSrBatch], rb[kSrBatch];
-      auto issue = [&](R (&buf)[kSrBatch], int32_t b0) {
-        const int32_t nb = min(kSrBatch, cn - b0);
-        if (colok) {
-#pragma unroll
-          for (int u = 0; u < kSrBatch; ++u)
-            if (u < nb)
-              buf[u] = load_raw<DT, V>(B, (int64_t)s_crd[b0 + u] * ldb + col);
-        }
-      };
-      auto consume = [&](const R (&buf)[kSrBatch], int32_t b0) {
-        const int32_t nb = min(kSrBatch, cn - b0);
-#pragma unroll
-        for (int u = 0; u < kSrBatch; ++u) {
-          if (u < nb) {
-            const int32_t pp = cs + b0 + u;
-            if (pp == re) next_row(pp);
-            if (colok) {
-              const float v = s_val[b0 + u];
-              float b[V];
-              unpack<DT, V>(buf[u], b);
-              if (comp) {
-#pragma unroll
-                for (int j = 0; j < V; ++j) neumaier_add(acc[j], cmp[j], v, b[j]);
-              } else {
-#pragma unroll
-                for (int j = 0; j < V; ++j) acc[j] = fmaf(v, b[j], acc[j]);
-              }
-            }
-          }
-        }
-      };
-      issue(ra, 0);
-      for (int32_t b0 = 0; b0 < cn; b0 += 2 * kSrBatch) {
-        if (b0 + kSrBatch < cn) issue(rb, b0 + kSrBatch);
-        consume(ra, b0);
-        if (b0 + 2 * kSrBatch < cn) issue(ra, b0 + 2 * kSrBatch);
-        if (b0 + kSrBatch < cn) consume(rb, b0 + kSrBatch);
-      }
-    }
-    wave_sync();
-  }
-  if (tok) flush();
+  eb_sr_body<DT, V>(a, blk, panel, smem);
 }
 
 template <int DT, int V>
@@ -227,174 +20,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_eb_pr(const KArgs a) {
   extern __shared__ int32_t smem[];
   int32_t blk, panel;
   if (!decode_block(a.panels, a.nblocks, blk, panel)) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int lq = a.lq;
-  const int32_t P = 64 >> lq;  // entry slots per pass
-  const int32_t t = blk * kWavesPerBlock + wave;
-  if (t >= a.ntiles) return;  // uniform over the wave
-  const int32_t q = lane & ((1 << lq) - 1);
-  const int32_t sl = lane >> lq;
-  constexpr int32_t CH = kPrChunk;
-  constexpr int32_t PW = kPrChunk + kPosSlack;
-  int32_t *s_crd = smem + wave * (4 * CH + kPosSlack);
-  float *s_val = reinterpret_cast<float *>(s_crd + CH);
-  int32_t *s_row = s_crd + 2 * CH;
-  int32_t *s_pos = s_crd + 3 * CH;
-
-  const int32_t *__restrict__ pos = a.pos;
-  const int32_t *__restrict__ crd = a.crd;
-  const void *__restrict__ vals = a.vals;
-  const void *__restrict__ B = a.B;
-  const int64_t ldb = a.ldb;
-  const int32_t col = ((panel << lq) + q) * V;
-  const bool colok = col < a.N;
-  const int32_t s = t * a.tile;
-  const int32_t e = (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile);
-  const int32_t r_hi = min(a.M - 1, a.part[t + 1]);
-  int32_t r = a.tile_row[t];
-  int32_t rbase = r, pw = 0;
-  auto Pos = [&](int32_t row) -> int32_t {
-    const int32_t i = row - rbase;
-    return i < pw ? s_pos[i] : pos[row];
-  };
-
-  float open[V];
-  zero_vec(open);
-  int32_t open_row = -1;
-  using R = typename RawT<DT, V>::type;
-
-  for (int32_t cs = s; cs < e; cs += CH) {
-    const int32_t cn = min(CH, e - cs);
-    for (int32_t i = lane; i < cn; i += 64) {
-      s_crd[i] = crd[cs + i];
-      s_val[i] = load_val<DT>(vals, cs + i);
-    }
-    rbase = r;
-    pw = min(PW, a.M + 1 - rbase);
-    for (int32_t i = lane; i < pw; i += 64) s_pos[i] = pos[rbase + i];
-    wave_sync();
-    // Row of every entry of the chunk: last j with pos[rbase + j] <= p.
-    for (int32_t i = lane; i < cn; i += 64) {
-      const int32_t p = cs + i;
-      int32_t lo = 0, hi = pw - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (s_pos[mid] <= p)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      s_row[i] = (lo == pw - 1) ? row_of(pos, rbase + lo, r_hi, p) : rbase + lo;
-    }
-    wave_sync();
-    for (int32_t pb = 0; pb < cn; pb += P * kPrPasses) {
-      int32_t rid[kPrPasses];
-      float vv[kPrPasses];
-      R raw[kPrPasses];
-#pragma unroll
-      for (int u = 0; u < kPrPasses; ++u) {
-        const int32_t i = pb + u * P + sl;
-        const bool ok = i < cn;
-        rid[u] = ok ? s_row[i] : INT_MAX;
-        vv[u] = ok ? s_val[i] : 0.0f;
-        if (ok && colok) raw[u] = load_raw<DT, V>(B, (int64_t)s_crd[i] * ldb + col);
-      }
-#pragma unroll
-      for (int u = 0; u < kPrPasses; ++u) {
-        const int32_t i0 = pb + u * P;
-        if (i0 >= cn) break;  // uniform over the wave
-        float prod[V];
-        zero_vec(prod);
-        if (rid[u] != INT_MAX && colok) {
-          float b[V];
-          unpack<DT, V>(raw[u], b);
-#pragma unroll
-          for (int j = 0; j < V; ++j) prod[j] = vv[u] * b[j];
-        }
-        const int32_t my = rid[u];
-        // Segmented suffix scan: slot sl accumulates slot sl+d of its row.
-        for (int32_t d = 1; d < P; d <<= 1) {
-          const int32_t tr = __shfl_down(my, d << lq, 64);
-          float tv[V];
-#pragma unroll
-          for (int j = 0; j < V; ++j) tv[j] = __shfl_down(prod[j], d << lq, 64);
-          if (sl + d < P && tr == my) {
-#pragma unroll
-            for (int j = 0; j < V; ++j) prod[j] += tv[j];
-          }
-        }
-        const int32_t prev = __shfl_up(my, 1 << lq, 64);
-        const bool head = my != INT_MAX && (sl == 0 || prev != my);
-        if (sl == 0 && my == open_row) {
-#pragma unroll
-          for (int j = 0; j < V; ++j) prod[j] = open[j] + prod[j];
-        }
-        const uint64_t hm = __ballot(head);
-        const int32_t last_head = (63 - __clzll((long long)hm)) >> lq;
-        const int32_t last_row = __shfl(my, last_head << lq, 64);
-        const int32_t pass_end = cs + min(i0 + P, cn);
-        const bool last_open = pass_end < e && Pos(last_row + 1) > pass_end;
-        if (head && colok && !(last_open && sl == last_head)) {
-          const bool before = Pos(my) < s;
-          const bool after = Pos(my + 1) > e;
-          float *dst = before  ? a.carry_head + (int64_t)t * a.N + col
-                       : after ? a.carry_tail + (int64_t)t * a.N + col
-                               : a.C + (int64_t)my * a.ldc + col;
-          store_vec<V>(dst, prod);
-        }
-        if (last_open) {
-#pragma unroll
-          for (int j = 0; j < V; ++j)
-            open[j] = __shfl(prod[j], (last_head << lq) + q, 64);
-          open_row = last_row;
-        } else {
-          open_row = -1;
-        }
-      }
-    }
-    r = s_row[cn - 1];
-    wave_sync();
-  }
+  eb_pr_body<DT, V>(a, blk, panel, smem);
 }
 
-// Per tile t (one lane group): zero the empty rows of its row slice, then, if
-// the tile's first row started in an earlier tile t0 and ends in t, write
-// C[r] = carry_tail[t0] + carry_head[t0+1] + ... + carry_head[t].
 template <int V>
 __global__ __launch_bounds__(kBlockThreads) void k_fixup(const KArgs a) {
   int32_t blk, panel;
   if (!decode_block(a.panels, a.nblocks, blk, panel)) return;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int lq = a.lq;
-  const int32_t Q = 1 << lq;
-  const int32_t q = lane & (Q - 1);
-  const int32_t t = ((blk * kWavesPerBlock + wave) << (6 - lq)) + (lane >> lq);
-  if (t >= a.ntiles) return;
-  {
-    const int64_t r0 = (int64_t)t * a.rows_per_slice;
-    const int64_t r1 = r0 + a.rows_per_slice < a.M ? r0 + a.rows_per_slice : a.M;
-    if (r0 < r1) zero_empty_rows<V>(a, (int32_t)r0, (int32_t)r1, q, Q, panel);
-  }
-  const int32_t col = ((panel << lq) + q) * V;
-  if (t == 0 || col >= a.N) return;
-  const int32_t *__restrict__ pos = a.pos;
-  const int32_t s = t * a.tile;
-  const int32_t e = (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile);
-  const int32_t r = a.tile_row[t];
-  const int32_t rs = pos[r], re = pos[r + 1];
-  if (!(rs < s && re <= e)) return;
-  const int32_t t0 = rs / a.tile;
-  float acc[V];
-  load_f32vec<V>(a.carry_tail + (int64_t)t0 * a.N + col, acc);
-  for (int32_t tt = t0 + 1; tt <= t; ++tt) {
-    float h[V];
-    load_f32vec<V>(a.carry_head + (int64_t)tt * a.N + col, h);
-#pragma unroll
-    for (int j = 0; j < V; ++j) acc[j] += h[j];
-  }
-  store_vec<V>(a.C + (int64_t)r * a.ldc + col, acc);
+  fixup_body<V>(a, blk, panel);
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_block_starts(

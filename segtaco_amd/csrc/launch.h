@@ -50,4 +50,26 @@ size_t eb_pr_lds_bytes(const KArgs &a);
 hipError_t launch_fill_zero(float *C, int32_t M, int32_t N, int64_t ldc,
                             hipStream_t s);
 
+// One product of a batch, as the batched kernels see it.
+struct BatchDesc {
+  KArgs a;
+  int32_t schedule;    // SEGTACO_RB_SR .. SEGTACO_EB_PR
+  int32_t vec;         // elements per lane per gather
+  int64_t unit_begin;  // first unit of this product in the persistent launch
+  int64_t fix_begin;   // first unit of this product in the batched fix-up
+};
+
+// Persistent launch over `total` units (prefix[i] = first unit of product i,
+// non-decreasing, n entries); workgroups take units from *counter, which the
+// launcher resets first. grid = persistent workgroups, lds = dynamic LDS.
+hipError_t launch_batch_main(int dt, const BatchDesc *d, int32_t n,
+                             const int64_t *prefix, int64_t total,
+                             unsigned long long *counter, int32_t grid,
+                             size_t lds, hipStream_t s);
+hipError_t launch_batch_fixup(const BatchDesc *d, int32_t n,
+                              const int64_t *fix_prefix, int64_t total,
+                              hipStream_t s);
+// Workgroups of the persistent launch that fit the device at once.
+hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid);
+
 }  // namespace segtaco
