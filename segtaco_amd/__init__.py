@@ -92,6 +92,29 @@ class BatchItem(ctypes.Structure):
                 ("plan", Plan), ("pos_host", _i32p)]
 
 
+class ServeConfig(ctypes.Structure):
+    """segtaco_serve_config_t."""
+    _fields_ = [("cache_bytes", ctypes.c_size_t), ("inflight_bytes", ctypes.c_size_t),
+                ("max_batch", ctypes.c_int32), ("streams", ctypes.c_int32),
+                ("dtype", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    """segtaco_serve_stats_t."""
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("rejected", ctypes.c_int64), ("ticks", ctypes.c_int64),
+                ("cache_hits", ctypes.c_int64), ("cache_misses", ctypes.c_int64),
+                ("evictions", ctypes.c_int64), ("uploaded_bytes", ctypes.c_int64),
+                ("mean_batch", ctypes.c_double), ("p50_us", ctypes.c_double),
+                ("p99_us", ctypes.c_double), ("max_us", ctypes.c_double),
+                ("mean_us", ctypes.c_double), ("gflops", ctypes.c_double),
+                ("wall_s", ctypes.c_double), ("max_concurrent_ticks", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: (getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
+
+
 class TacoTensor(ctypes.Structure):
     """taco_tensor_t (taco_tensor_t.h:6-15), identical layout."""
     _fields_ = [("order", ctypes.c_int32), ("dimensions", _i32p), ("csize", ctypes.c_int32),
@@ -209,6 +232,34 @@ def lib() -> ctypes.CDLL:
     L.segtaco_batch_workspace_bytes.argtypes = [_vp]
     L.segtaco_batch_destroy.restype = ctypes.c_int32
     L.segtaco_batch_destroy.argtypes = [_vp]
+    L.segtaco_server_create.restype = ctypes.c_int32
+    L.segtaco_server_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(_vp)]
+    L.segtaco_server_add_matrix.restype = ctypes.c_int32
+    L.segtaco_server_add_matrix.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            _i32p, _i32p, _vp, ctypes.POINTER(ctypes.c_int32)]
+    L.segtaco_server_submit.restype = ctypes.c_int32
+    L.segtaco_server_submit.argtypes = [_vp, ctypes.c_int32, _vp, ctypes.c_int64, ctypes.c_int32,
+                                        _vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+    L.segtaco_server_wait.restype = ctypes.c_int32
+    L.segtaco_server_wait.argtypes = [_vp, ctypes.c_int64, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_int32)]
+    L.segtaco_server_drain.restype = ctypes.c_int32
+    L.segtaco_server_drain.argtypes = [_vp, ctypes.c_double]
+    L.segtaco_server_replay.restype = ctypes.c_int32
+    L.segtaco_server_replay.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp),
+                                        ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp),
+                                        ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.c_double]
+    L.segtaco_server_stats.restype = ctypes.c_int32
+    L.segtaco_server_stats.argtypes = [_vp, ctypes.POINTER(ServeStats)]
+    L.segtaco_server_reset_stats.restype = ctypes.c_int32
+    L.segtaco_server_reset_stats.argtypes = [_vp]
+    L.segtaco_server_plan.restype = ctypes.c_int32
+    L.segtaco_server_plan.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Plan)]
+    L.segtaco_server_destroy.restype = ctypes.c_int32
+    L.segtaco_server_destroy.argtypes = [_vp]
     _LIB = L
     return L
 
@@ -514,6 +565,97 @@ class Batch:
     def destroy(self) -> None:
         if self.handle:
             lib().segtaco_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Continuous-batching server (include/segtaco/serve.h)
+# ---------------------------------------------------------------------------
+
+class Server:
+    """Owns a segtaco_server_t. B and C of every request are device pointers."""
+
+    def __init__(self, cache_bytes: int, inflight_bytes: int, max_batch: int = 64,
+                 streams: int = 2, dtype: str | int = "f32"):
+        self.dtype = DTYPES[dtype] if isinstance(dtype, str) else dtype
+        cfg = ServeConfig(cache_bytes, inflight_bytes, max_batch, streams, self.dtype, 0)
+        h = _vp()
+        check(lib().segtaco_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+
+    def add_matrix(self, pos: np.ndarray, crd: np.ndarray, vals, K: int) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.int32)
+        crd = np.ascontiguousarray(crd, dtype=np.int32)
+        if crd.size == 0:
+            crd = np.zeros(1, dtype=np.int32)
+        M = len(pos) - 1
+        vptr = None
+        if vals is not None:
+            v = (np.ascontiguousarray(vals, dtype=np.float32) if self.dtype == DT_F32
+                 else to_half(self.dtype, vals))
+            vptr = v.ctypes.data
+        mid = ctypes.c_int32()
+        check(lib().segtaco_server_add_matrix(self.handle, M, K, int(pos[M]) if M else 0,
+                                              pos.ctypes.data_as(_i32p),
+                                              crd.ctypes.data_as(_i32p), vptr,
+                                              ctypes.byref(mid)))
+        return int(mid.value)
+
+    def submit(self, mid: int, B_ptr: int, ldb: int, N: int, C_ptr: int, ldc: int) -> int:
+        rid = ctypes.c_int64()
+        check(lib().segtaco_server_submit(self.handle, mid, B_ptr, ldb, N, C_ptr, ldc,
+                                          ctypes.byref(rid)))
+        return int(rid.value)
+
+    def wait(self, rid: int, timeout: float = -1.0) -> int:
+        st = ctypes.c_int32()
+        check(lib().segtaco_server_wait(self.handle, rid, timeout, ctypes.byref(st)))
+        return int(st.value)
+
+    def drain(self, timeout: float = -1.0) -> None:
+        check(lib().segtaco_server_drain(self.handle, timeout))
+
+    def replay(self, arrival_us, mids, B_ptrs, ldbs, Ns, C_ptrs, ldcs, timeout: float = 600.0):
+        n = len(mids)
+        arr = np.ascontiguousarray(arrival_us, dtype=np.float64)
+        m = np.ascontiguousarray(mids, dtype=np.int32)
+        lb = np.ascontiguousarray(ldbs, dtype=np.int64)
+        nn = np.ascontiguousarray(Ns, dtype=np.int32)
+        lc = np.ascontiguousarray(ldcs, dtype=np.int64)
+        bp = (_vp * n)(*B_ptrs)
+        cp = (_vp * n)(*C_ptrs)
+        ids = np.zeros(n, dtype=np.int64)
+        check(lib().segtaco_server_replay(
+            self.handle, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            m.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), bp,
+            lb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            nn.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cp,
+            lc.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), timeout))
+        return ids
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        check(lib().segtaco_server_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        check(lib().segtaco_server_reset_stats(self.handle))
+
+    def plan(self, mid: int, N: int) -> Plan:
+        p = Plan()
+        check(lib().segtaco_server_plan(self.handle, mid, N, ctypes.byref(p)))
+        return p
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().segtaco_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -34,7 +34,7 @@ COMMON = [
 
 LIB_SOURCES = [
     "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
-    "gen.cpp", "autotune.cpp", "batch.cpp",
+    "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp",
 ]
 
 

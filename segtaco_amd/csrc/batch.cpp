@@ -76,6 +76,62 @@ int32_t tune_item(const segtaco_batch_item_t &it, int32_t dt,
 
 }  // namespace
 
+namespace segtaco {
+
+void fill_batch_desc(const segtaco_csr_t &A, const void *B, int64_t ldb,
+                     float *C, int64_t ldc, int32_t N,
+                     const segtaco_plan_t &plan, BatchDesc *D, int64_t *units,
+                     int64_t *fix_units, size_t *lds, EbLayout *lay) {
+  std::memset(D, 0, sizeof(*D));
+  KArgs &a = D->a;
+  a.M = A.M;
+  a.K = A.K;
+  a.N = N;
+  a.nnz = A.nnz;
+  a.pos = A.pos;
+  a.crd = A.crd;
+  a.vals = A.vals;
+  a.B = B;
+  a.ldb = ldb;
+  a.C = C;
+  a.ldc = ldc;
+  a.lq = ilog2(plan.col_lanes);
+  a.lg = ilog2(plan.group);
+  a.tile = plan.tile;
+  a.comp_threshold = plan.comp_threshold == 0 ? INT_MAX : plan.comp_threshold;
+  a.panels = (int32_t)ceil_div(N, (int64_t)plan.col_lanes * plan.vec);
+  D->schedule = plan.schedule;
+  D->vec = plan.vec;
+  *fix_units = 0;
+  *lds = 0;
+  switch (plan.schedule) {
+    case SEGTACO_RB_SR:
+      a.nblocks = (int32_t)ceil_div(
+          A.M, (int64_t)kWavesPerBlock * (64 / plan.col_lanes) * plan.group);
+      break;
+    case SEGTACO_RB_PR:
+      a.nblocks = (int32_t)ceil_div(
+          A.M, (int64_t)(64 / (plan.col_lanes * plan.group)) * kWavesPerBlock);
+      break;
+    default: {
+      *lay = eb_layout(A.nnz, N, plan.tile);
+      a.ntiles = lay->ntiles;
+      a.rows_per_slice = (int32_t)ceil_div(A.M, lay->ntiles);
+      const int64_t tiles_per_group_block =
+          (int64_t)kWavesPerBlock * (64 / plan.col_lanes);
+      a.nblocks = plan.schedule == SEGTACO_EB_SR
+                      ? (int32_t)ceil_div(lay->ntiles, tiles_per_group_block)
+                      : (int32_t)ceil_div(lay->ntiles, kWavesPerBlock);
+      *lds = plan.schedule == SEGTACO_EB_SR ? eb_sr_lds_bytes(a) : eb_pr_lds_bytes(a);
+      *fix_units = ceil_div(lay->ntiles, tiles_per_group_block) * a.panels;
+      break;
+    }
+  }
+  *units = (int64_t)a.nblocks * a.panels;
+}
+
+}  // namespace segtaco
+
 extern "C" {
 
 int32_t segtaco_batch_create(const segtaco_batch_item_t *items, int32_t n,
@@ -141,56 +197,18 @@ int32_t segtaco_batch_create(const segtaco_batch_item_t *items, int32_t n,
       }
     }
     b->plans[i] = plan;
-    KArgs &a = D.a;
-    a.M = A.M;
-    a.K = A.K;
-    a.N = it.N;
-    a.nnz = A.nnz;
-    a.pos = A.pos;
-    a.crd = A.crd;
-    a.vals = A.vals;
-    a.B = it.B;
-    a.ldb = it.ldb;
-    a.C = it.C;
-    a.ldc = it.ldc;
-    a.lq = ilog2(plan.col_lanes);
-    a.lg = ilog2(plan.group);
-    a.tile = plan.tile;
-    a.comp_threshold = plan.comp_threshold == 0 ? INT_MAX : plan.comp_threshold;
-    a.panels = (int32_t)ceil_div(it.N, (int64_t)plan.col_lanes * plan.vec);
-    D.schedule = plan.schedule;
-    D.vec = plan.vec;
-    int64_t fix_units = 0;
-    switch (plan.schedule) {
-      case SEGTACO_RB_SR:
-        a.nblocks = (int32_t)ceil_div(
-            A.M, (int64_t)kWavesPerBlock * (64 / plan.col_lanes) * plan.group);
-        break;
-      case SEGTACO_RB_PR:
-        a.nblocks = (int32_t)ceil_div(
-            A.M, (int64_t)(64 / (plan.col_lanes * plan.group)) * kWavesPerBlock);
-        break;
-      default: {
-        lay[i] = eb_layout(A.nnz, it.N, plan.tile);
-        ws_off[i] = ws_total;
-        ws_total += align_up(lay[i].total, 256);
-        a.ntiles = lay[i].ntiles;
-        a.rows_per_slice = (int32_t)ceil_div(A.M, lay[i].ntiles);
-        a.nblocks = plan.schedule == SEGTACO_EB_SR
-                        ? (int32_t)ceil_div(lay[i].ntiles,
-                                            (int64_t)kWavesPerBlock * (64 / plan.col_lanes))
-                        : (int32_t)ceil_div(lay[i].ntiles, kWavesPerBlock);
-        const size_t lds = plan.schedule == SEGTACO_EB_SR ? eb_sr_lds_bytes(a)
-                                                          : eb_pr_lds_bytes(a);
-        if (lds > b->lds) b->lds = lds;
-        fix_units = ceil_div(lay[i].ntiles, (int64_t)kWavesPerBlock * (64 / plan.col_lanes)) *
-                    a.panels;
-        break;
-      }
+    int64_t units = 0, fix_units = 0;
+    size_t lds = 0;
+    fill_batch_desc(A, it.B, it.ldb, it.C, it.ldc, it.N, plan, &D, &units,
+                    &fix_units, &lds, &lay[i]);
+    if (lds > b->lds) b->lds = lds;
+    if (plan.schedule >= SEGTACO_EB_SR) {
+      ws_off[i] = ws_total;
+      ws_total += align_up(lay[i].total, 256);
     }
     D.unit_begin = b->prefix[i];
     D.fix_begin = b->fix_prefix[i];
-    b->prefix[i + 1] = b->prefix[i] + (int64_t)a.nblocks * a.panels;
+    b->prefix[i + 1] = b->prefix[i] + units;
     b->fix_prefix[i + 1] = b->fix_prefix[i] + fix_units;
   }
   if (st != SEGTACO_OK) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "segtaco/spmm.h"
+
 namespace segtaco {
 
 constexpr int kBlockThreads = 256;  // 4 waves of 64 lanes
@@ -69,6 +71,14 @@ hipError_t launch_batch_main(int dt, const BatchDesc *d, int32_t n,
 hipError_t launch_batch_fixup(const BatchDesc *d, int32_t n,
                               const int64_t *fix_prefix, int64_t total,
                               hipStream_t s);
+// Fills D (and the product's unit counts, dynamic LDS need and, for
+// element-balanced plans, its workspace layout) for a resolved plan. The
+// element-balanced pointers (part, tile_row, carries) are left null.
+struct EbLayout;
+void fill_batch_desc(const segtaco_csr_t &A, const void *B, int64_t ldb,
+                     float *C, int64_t ldc, int32_t N,
+                     const segtaco_plan_t &plan, BatchDesc *D, int64_t *units,
+                     int64_t *fix_units, size_t *lds, EbLayout *lay);
 // Workgroups of the persistent launch that fit the device at once.
 hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid);
 

@@ -1,0 +1,172 @@
+"""GPU checks of the continuous-batching server.
+
+* Every request's result matches the oracle and is bit-identical to the same
+  product run alone (single call with the plan the server uses).
+* A cache smaller than the working set evicts and still computes correctly.
+* Ticks overlap on the device (two streams busy at once under a burst).
+* A request that cannot fit the configured capacities is rejected, not hung.
+* A Poisson trace replays to completion with latency statistics.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import segtaco_amd as S
+from gpu_util import DeviceCSR, dense_dev, run_spmm, stream_ptr, torch
+
+pytestmark = pytest.mark.gpu
+
+MiB = 1 << 20
+
+
+def _pool(seed, n, dtype="f32"):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for i in range(n):
+        M = int(rng.integers(300, 8000))
+        kind = [S.GEN_POWERLAW, S.GEN_UNIFORM, S.GEN_BANDED][i % 3]
+        pos, crd, vals = S.gen_csr(kind, 1000 + i, M, M, mean=float(rng.uniform(4, 48)),
+                                   bw=int(rng.integers(2, 20)))
+        mats.append((pos, crd, vals, M))
+    return mats
+
+
+def _requests(rng, mats, n, Ns=(16, 32, 64)):
+    reqs = []
+    for _ in range(n):
+        mi = int(rng.integers(0, len(mats)))
+        N = int(rng.choice(Ns))
+        K = mats[mi][3]
+        reqs.append((mi, N, rng.uniform(-1, 1, (K, N)).astype(np.float32)))
+    return reqs
+
+
+def _check_results(server, mats, reqs, dev, dtype):
+    for (mi, N, B), (Bd, C) in zip(reqs, dev):
+        pos, crd, vals, K = mats[mi]
+        Cg = C.cpu().numpy()
+        vr = vals if dtype == "f32" else O.round_to(dtype, vals)[1]
+        Br = B if dtype == "f32" else O.round_to(dtype, B)[1]
+        err = O.scaled_err(Cg, O.spmm_f64(pos, crd, vr, Br), O.spmm_absbound(pos, crd, vr, Br))
+        assert err <= 1e-5, f"matrix {mi} N {N}: {err:.3e}"
+        A = DeviceCSR(pos, crd, vals, K, dtype=dtype)
+        alone = run_spmm(A, Bd, N, server.plan(mi, N), b_dtype=dtype)[: A.M].cpu().numpy()
+        assert np.array_equal(alone, Cg), f"matrix {mi} N {N}: mixed != alone"
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_mixed_requests_equal_alone(dtype):
+    rng = np.random.default_rng(5)
+    mats = _pool(7, 12, dtype)
+    server = S.Server(cache_bytes=256 * MiB, inflight_bytes=256 * MiB, max_batch=8, streams=2,
+                      dtype=dtype)
+    ids = [server.add_matrix(p, c, v, K) for p, c, v, K in mats]
+    assert ids == list(range(len(mats)))
+    reqs = _requests(rng, mats, 48)
+    dev = []
+    for mi, N, B in reqs:
+        M = mats[mi][3]
+        Bd = dense_dev(B, dtype)
+        C = torch().full((M, N), float("nan"), dtype=torch().float32, device="cuda")
+        dev.append((Bd, C))
+    torch().cuda.synchronize()
+    rids = [server.submit(mi, Bd.data_ptr(), N, N, C.data_ptr(), N)
+            for (mi, N, _), (Bd, C) in zip(reqs, dev)]
+    for rid in rids:
+        assert server.wait(rid, 120.0) == S.OK
+    st = server.stats()
+    print(f"[serve] {st}", flush=True)
+    assert st["completed"] == len(reqs) and st["ticks"] >= 1
+    _check_results(server, mats, reqs, dev, dtype)
+    server.destroy()
+
+
+def test_eviction_under_small_cache():
+    rng = np.random.default_rng(9)
+    mats = _pool(11, 10)
+    # CSR bytes (+ tile rows) of the largest matrix; the cache holds ~2 of 10
+    biggest = max(4 * (K + 1) + 8 * int(p[-1]) + 8 * (int(p[-1]) // 64 + 2) + 4096
+                  for p, _, _, K in mats)
+    cache = int(2.2 * biggest)
+    server = S.Server(cache_bytes=cache, inflight_bytes=128 * MiB, max_batch=4, streams=2)
+    for p, c, v, K in mats:
+        server.add_matrix(p, c, v, K)
+    reqs = [(i % len(mats), 32, rng.uniform(-1, 1, (mats[i % len(mats)][3], 32)).astype(np.float32))
+            for i in range(30)]
+    dev = []
+    for mi, N, B in reqs:
+        dev.append((dense_dev(B), torch().full((mats[mi][3], N), float("nan"),
+                                               dtype=torch().float32, device="cuda")))
+    torch().cuda.synchronize()
+    for (mi, N, _), (Bd, C) in zip(reqs, dev):
+        server.submit(mi, Bd.data_ptr(), N, N, C.data_ptr(), N)
+    server.drain(120.0)
+    st = server.stats()
+    print(f"[serve evict] {st}", flush=True)
+    assert st["completed"] == 30 and st["evictions"] > 0 and st["cache_misses"] > len(mats)
+    _check_results(server, mats, reqs, dev, "f32")
+    server.destroy()
+
+
+def test_ticks_overlap_and_capacity_rejection():
+    rng = np.random.default_rng(13)
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 3, 60000, 60000, mean=24)
+    server = S.Server(cache_bytes=64 * MiB, inflight_bytes=512 * MiB, max_batch=2, streams=2)
+    mid = server.add_matrix(pos, crd, vals, 60000)
+    B = dense_dev(rng.uniform(-1, 1, (60000, 64)).astype(np.float32))
+    Cs = [torch().empty((60000, 64), dtype=torch().float32, device="cuda") for _ in range(16)]
+    torch().cuda.synchronize()
+    for C in Cs:
+        server.submit(mid, B.data_ptr(), 64, 64, C.data_ptr(), 64)
+    server.drain(120.0)
+    st = server.stats()
+    print(f"[serve overlap] {st}", flush=True)
+    assert st["completed"] == 16 and st["max_concurrent_ticks"] == 2
+    ref = Cs[0].cpu().numpy()
+    for C in Cs[1:]:
+        assert np.array_equal(C.cpu().numpy(), ref)
+    # a matrix larger than the whole cache can never be admitted
+    big_pos, big_crd, big_vals = S.gen_csr(S.GEN_UNIFORM, 4, 200000, 200000, mean=64)
+    big = server.add_matrix(big_pos, big_crd, big_vals, 200000)
+    Bb = dense_dev(rng.uniform(-1, 1, (200000, 16)).astype(np.float32))
+    Cb = torch().empty((200000, 16), dtype=torch().float32, device="cuda")
+    rid = server.submit(big, Bb.data_ptr(), 16, 16, Cb.data_ptr(), 16)
+    assert server.wait(rid, 10.0) == 5  # SEGTACO_ERR_CAPACITY
+    server.destroy()
+
+
+def test_poisson_replay_latency_stats():
+    rng = np.random.default_rng(17)
+    mats = _pool(19, 16)
+    server = S.Server(cache_bytes=512 * MiB, inflight_bytes=512 * MiB, max_batch=32, streams=2)
+    for p, c, v, K in mats:
+        server.add_matrix(p, c, v, K)
+    n = 300
+    arrivals = np.cumsum(rng.exponential(1e6 / 2000.0, n))  # lambda = 2000 / s
+    pop = 1.0 / np.arange(1, len(mats) + 1)
+    mids = rng.choice(len(mats), n, p=pop / pop.sum())
+    Ns = rng.choice([16, 32, 64, 128], n)
+    Bs = {}
+    Cs = []
+    for mi, N in zip(mids, Ns):
+        K = mats[mi][3]
+        if (mi, N) not in Bs:
+            Bs[(mi, N)] = dense_dev(rng.uniform(-1, 1, (K, N)).astype(np.float32))
+        Cs.append(torch().empty((K, N), dtype=torch().float32, device="cuda"))
+    torch().cuda.synchronize()
+    ids = server.replay(arrivals, mids, [Bs[(m, N)].data_ptr() for m, N in zip(mids, Ns)],
+                        Ns, Ns, [C.data_ptr() for C in Cs], Ns, timeout=120.0)
+    assert len(ids) == n
+    st = server.stats()
+    print(f"[serve poisson] {st}", flush=True)
+    assert st["completed"] == n and st["rejected"] == 0
+    assert 0 < st["p50_us"] <= st["p99_us"] <= st["max_us"]
+    # spot-check a few results
+    for k in (0, n // 2, n - 1):
+        mi, N = int(mids[k]), int(Ns[k])
+        pos, crd, vals, K = mats[mi]
+        B = Bs[(mi, N)].cpu().numpy()
+        err = O.scaled_err(Cs[k].cpu().numpy(), O.spmm_f64(pos, crd, vals, B),
+                           O.spmm_absbound(pos, crd, vals, B))
+        assert err <= 1e-5
+    server.destroy()
