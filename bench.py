@@ -11,11 +11,14 @@ plan). Every schedule is timed; `value` is the fastest one's GFLOP/s
 Also reported (not the headline): the precision workloads — the same
 power-law A with N = 256 stored FP32 / BF16 / FP16, and a banded
 1048576 x 1048576 A (|i - j| <= 32) with N = 256 in BF16 — each with FP32
-accumulation; and the reference's Eigen CPU path (oracle/_ref) timed on this
-host's cores on the headline matrix as the CPU baseline.
+accumulation; rocSPARSE's CSR SpMM algorithms on the headline operands (the
+in-box vendor library, compute stage only); batched and served legs; and the
+reference's Eigen CPU path (oracle/_ref) timed on this host's cores on the
+headline matrix as the CPU baseline.
 
-Roofline: compulsory bytes per step 4(M+1) + nnz*(4+s_v) + K_touched*N*s_B
-+ M*N*4 over the step's event time, against 8 TB/s.
+Roofline: compulsory bytes per launch 4(M+1) + nnz*(4+s_v) + K_touched*N*s_B
++ M*N*4 over the fastest schedule's main-kernel time (HIP events around that
+kernel alone, on its launch stream), against 8 TB/s.
 
     python bench.py [--gpus 1] [--steps 25] [--warmup 5]
 """
@@ -107,6 +110,74 @@ def time_plan(torch, A, Bd, b_dtype, C, N, plan, steps, warmup, stream):
     return rp, total / steps, per
 
 
+def main_kernel_ms(torch, ops, N, plans, name, reps, stream):
+    """(partition, main kernel, fix-up) mean ms of plan `name` on the headline
+    operands, each stage bracketed by HIP events on the launch stream."""
+    plan = dict(plans)[name]
+    Bd = ops.dense(3, N, "f32")
+    C = torch.empty((ops.M, N), dtype=torch.float32, device="cuda")
+    A = ops.desc("f32")
+    rp = S.plan_resolve(A, N, S.DT_F32, plan)
+    ws_bytes = S.workspace_bytes(A, N, rp)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    st = S.spmm_stage_times(A, Bd.data_ptr(), S.DT_F32, N, N, C.data_ptr(), N, rp,
+                            ws.data_ptr(), ws_bytes, stream.cuda_stream, reps=max(reps, 1))
+    log(f"{name} stages (ms): partition {st[0]:.4f}  main {st[1]:.4f}  fixup {st[2]:.4f}")
+    del Bd, C, ws
+    return st
+
+
+VENDOR_ALGS = {0: "default", 1: "csr", 2: "csr_row_split", 3: "csr_nnz_split",
+               4: "csr_merge_path"}
+
+
+def vendor_leg(torch, ops, N, reps, stream):
+    """rocSPARSE CSR SpMM on the headline operands (compute stage timed with
+    HIP events), one line per algorithm; an algorithm that refuses the
+    operands is reported with its error."""
+    Bd = ops.dense(3, N, "f32")
+    C = torch.empty((ops.M, N), dtype=torch.float32, device="cuda")
+    A = ops.desc("f32")
+    flops = 2.0 * ops.nnz * N
+    res = {}
+    for alg, name in VENDOR_ALGS.items():
+        try:
+            for _ in range(2):
+                S.vendor_spmm(A, Bd.data_ptr(), N, N, C.data_ptr(), N, alg, stream.cuda_stream,
+                              timed=True)
+            times = [S.vendor_spmm(A, Bd.data_ptr(), N, N, C.data_ptr(), N, alg,
+                                   stream.cuda_stream, timed=True) for _ in range(max(reps, 1))]
+            ms = statistics.median(times)
+            res[name] = {"ms": round(ms, 5), "ms_min": round(min(times), 5),
+                         "gflops": round(flops / (ms * 1e-3) / 1e9, 2)}
+            log(f"rocSPARSE {name}: {ms*1e3:.1f} us  {res[name]['gflops']:.1f} GFLOP/s")
+        except S.SegtacoError as e:
+            res[name] = {"error": str(e)}
+            log(f"rocSPARSE {name}: {e}")
+    del Bd, C
+    torch.cuda.empty_cache()
+    return res
+
+
+# The rocprofv3 --pmc summary of the headline workload carries one line
+# "traffic_bytes_per_launch <kernel>: <int>" per kernel (FETCH_SIZE x2 +
+# WRITE_SIZE x1, the factors measured on plain device copies of known size).
+TRAFFIC_FILE = "spmm_powerlaw_262144_n128_f32_hbm_traffic.txt"
+
+
+def profiled_traffic(name, vec=4):
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch k_{name}<0, {vec}>:"
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                return int(line.split(":", 1)[1].strip())
+    return None
+
+
 def parse_plans(spec: str):
     plans = []
     for item in spec.split(";"):
@@ -138,6 +209,169 @@ def best_form(res):
     b = min(res, key=lambda k: res[k]["ms"])
     return {"schedule": b, "ms": round(res[b]["ms"], 5), "gflops": round(res[b]["gflops"], 2),
             "hbm_frac": round(res[b]["hbm_frac"], 4)}
+
+
+def cfg4_set(n, seed=41):
+    """The batching workload: n CSR matrices, M ~ U[1000, 200000] (square),
+    log-normal row lengths with mean ~ U[8, 64] (sigma 1.5, cap 8192), each
+    with its own N drawn from {16, 32, 64, 128}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t0 = time.time()
+    for i in range(n):
+        M = int(rng.integers(1000, 200001))
+        mean = float(rng.uniform(8, 64))
+        N = int(rng.choice([16, 32, 64, 128]))
+        pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 5000 + i, M, M, mean=mean)
+        out.append((pos, crd, vals, M, N))
+    log(f"generated {n} matrices, {sum(int(m[0][-1]) for m in out)} nnz ({time.time()-t0:.1f}s)")
+    return out
+
+
+def _dense_on_device(torch, rows, N, dtype, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    B = torch.rand((rows, N), device="cuda", generator=g) * 2 - 1
+    return B if dtype == "f32" else B.to(torch.bfloat16 if dtype == "bf16" else torch.float16)
+
+
+def batch_leg(torch, mats, dtype, steps, warmup, stream):
+    """All products of the set in one persistent launch (graph replay); plans
+    from the autotuner (measured once per row-statistics signature)."""
+    keep, items = [], []
+    flops = 0.0
+    for i, (pos, crd, vals, M, N) in enumerate(mats):
+        d_pos = torch.from_numpy(pos).cuda()
+        d_crd = torch.from_numpy(crd).cuda()
+        v = vals if dtype == "f32" else S.to_half(dtype, vals).view(np.int16)
+        d_val = torch.from_numpy(v).cuda()
+        B = _dense_on_device(torch, M, N, dtype, 77 + i)
+        C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+        A = S.csr_desc(M, M, int(pos[-1]), d_pos.data_ptr(), d_crd.data_ptr(),
+                       d_val.data_ptr(), S.DTYPES[dtype])
+        keep.append((d_pos, d_crd, d_val, B, C, A))
+        items.append(S.batch_item(A, B.data_ptr(), N, C.data_ptr(), N, N, None, pos))
+        flops += 2.0 * int(pos[-1]) * N
+    # measured tuning, once per signature
+    t0 = time.time()
+    seen = set()
+    for (pos, crd, vals, M, N), (_, _, _, B, C, A) in zip(mats, keep):
+        st = S.row_stats(pos, M)
+        sig = S.tune_signature(st, N, dtype)
+        if sig in seen:
+            continue
+        seen.add(sig)
+        S.tune_measure(A, st, B.data_ptr(), S.DTYPES[dtype], N, N, C.data_ptr(), N, reps=2,
+                       stream=stream.cuda_stream)
+    log(f"tuned {len(seen)} signatures ({time.time()-t0:.1f}s)")
+    batch = S.Batch(items, dtype, stream.cuda_stream)
+    sched_hist = {}
+    for i in range(len(mats)):
+        name = ["rb_sr", "rb_pr", "eb_sr", "eb_pr"][batch.plan(i).schedule]
+        sched_hist[name] = sched_hist.get(name, 0) + 1
+    batch.capture()
+    for _ in range(warmup):
+        batch.replay(stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        batch.replay(stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    # the same products one call at a time, same plans
+    ws_max = 0
+    plans = [batch.plan(i) for i in range(len(mats))]
+    for (pos, _, _, M, N), k, p in zip(mats, keep, plans):
+        ws_max = max(ws_max, S.workspace_bytes(k[5], N, S.plan_resolve(k[5], N, S.DTYPES[dtype], p)))
+    ws = torch.empty(max(ws_max, 1), dtype=torch.uint8, device="cuda")
+
+    def seq():
+        for (pos, _, _, M, N), k, p in zip(mats, keep, plans):
+            S.spmm(k[5], k[3].data_ptr(), S.DTYPES[dtype], N, N, k[4].data_ptr(), N, p,
+                   ws.data_ptr(), ws_max, stream.cuda_stream)
+    seq()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    seq()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_seq = e0.elapsed_time(e1)
+    out = {"products": len(mats), "nnz": int(sum(int(m[0][-1]) for m in mats)),
+           "units": batch.units(), "schedules": sched_hist,
+           "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 2),
+           "sequential_ms": round(ms_seq, 4),
+           "sequential_gflops": round(flops / (ms_seq * 1e-3) / 1e9, 2)}
+    log(f"batched {dtype}: {ms:.3f} ms/launch {out['gflops']} GFLOP/s "
+        f"(one call per product: {ms_seq:.3f} ms) {sched_hist}")
+    batch.destroy()
+    del keep, items, ws
+    torch.cuda.empty_cache()
+    return out
+
+
+def serve_leg(torch, mats, dtype, seconds, lam, headline):
+    """Continuous batching over the set: Poisson arrivals at `lam` / s with
+    Zipf(1) popularity (latency under load), then a saturating burst
+    (capacity), then the headline power-law matrix alone at N = 128."""
+    GiB = 1 << 30
+    server = S.Server(cache_bytes=64 * GiB, inflight_bytes=16 * GiB, max_batch=32, streams=2,
+                      dtype=dtype)
+    t0 = time.time()
+    for pos, crd, vals, M, N in mats:
+        server.add_matrix(pos, crd, vals, M)
+    hid = server.add_matrix(headline.pos, headline.crd, headline.vals, headline.K)
+    log(f"registered {len(mats) + 1} matrices ({time.time()-t0:.1f}s)")
+    # Distinct B and C per request slot: at most max_batch * streams = 64
+    # requests are on the device at once, so no two running products share B.
+    Kmax = max(max(m[3] for m in mats), headline.K)
+    bring = [_dense_on_device(torch, Kmax, 128, dtype, 500 + i) for i in range(64)]
+    ring = [torch.empty((Kmax, 128), dtype=torch.float32, device="cuda") for _ in range(64)]
+    rng = np.random.default_rng(7)
+    pop = 1.0 / np.arange(1, len(mats) + 1)
+    pop /= pop.sum()
+
+    def trace(n, rate):
+        arr = np.cumsum(rng.exponential(1e6 / rate, n)) if rate else np.zeros(n)
+        mids = rng.choice(len(mats), n, p=pop)
+        Ns = np.array([mats[m][4] for m in mids], np.int32)
+        return arr, mids, Ns
+
+    def run(arr, mids, Ns):
+        server.replay(arr, mids, [bring[i % len(bring)].data_ptr() for i in range(len(mids))],
+                      Ns, Ns, [ring[i % len(ring)].data_ptr() for i in range(len(mids))], Ns,
+                      600.0)
+
+    # warm the cache (every matrix once), then measure under Poisson load
+    run(np.zeros(len(mats)), np.arange(len(mats)), np.array([m[4] for m in mats], np.int32))
+    server.reset_stats()
+    n = int(seconds * lam)
+    run(*trace(n, lam))
+    poisson = server.stats()
+    server.reset_stats()
+    run(*trace(2000, 0))
+    burst = server.stats()
+    server.reset_stats()
+    run(np.zeros(200), np.full(200, hid), np.full(200, 128, np.int32))
+    head = server.stats()
+    server.destroy()
+    del bring, ring
+    torch.cuda.empty_cache()
+
+    def pick(st):
+        return {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()
+                if k in ("completed", "rejected", "ticks", "mean_batch", "p50_us", "p99_us",
+                         "max_us", "gflops", "cache_misses", "evictions",
+                         "max_concurrent_ticks")}
+    out = {"dtype": dtype, "lambda_per_s": lam, "requests": n,
+           "poisson": pick(poisson), "saturated": pick(burst),
+           "powerlaw_n128_saturated": pick(head),
+           "latency_scope": "submit -> completion on the device (B, C device-resident)"}
+    log(f"served {dtype}: p50 {poisson['p50_us']:.0f} us p99 {poisson['p99_us']:.0f} us at "
+        f"{lam}/s; saturated {burst['gflops']:.0f} GFLOP/s; power-law N=128 "
+        f"{head['gflops']:.0f} GFLOP/s")
+    return out
 
 
 def cpu_baseline(ops, N, flops, budget_s=10.0):
@@ -178,6 +412,9 @@ def main():
                          "each 'schedule[:key=value[,key=value]]', e.g. 'eb_sr:tile=256'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline leg only")
+    ap.add_argument("--batch-matrices", type=int, default=512,
+                    help="products in the batching / serving workload (0 skips both)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run two plain 128 MiB device copies (known bytes) so that "
                          "rocprofv3 --pmc byte counters can be scaled")
@@ -214,7 +451,9 @@ def main():
     r = head[best]
     ms = r["ms"]
     flops = 2.0 * pl.nnz * N
-    achieved = cbytes / (ms * 1e-3) / 1e9
+    stages = main_kernel_ms(torch, pl, N, plans, best, args.steps, stream)
+    achieved = cbytes / (stages[1] * 1e-3) / 1e9
+    traffic = profiled_traffic(best, r["plan"]["vec"])
     out = {
         "metric": "SpMM GFLOP/s (power-law CSR, FP32)",
         "value": round(r["gflops"], 2),
@@ -233,14 +472,27 @@ def main():
                    "schedule": best, "plan": r["plan"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "note": ("achieved = compulsory bytes per step / event time per step "
-                              f"(compulsory {cbytes} B: pos + crd + vals + touched B rows + C)")},
+                     "traffic": traffic,
+                     "traffic_frac": (round(traffic / (stages[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                      if traffic else None),
+                     "kernel": f"{best} main kernel",
+                     "kernel_ms": round(stages[1], 5),
+                     "stage_ms": {"partition": round(stages[0], 5), "main": round(stages[1], 5),
+                                  "fixup": round(stages[2], 5)},
+                     "note": ("achieved = compulsory bytes per launch / mean main-kernel HIP-event "
+                              f"time (compulsory {cbytes} B: pos + crd + vals + touched B rows + "
+                              "C); traffic = rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 per launch "
+                              "from the committed profile of this kernel; traffic_frac = traffic / "
+                              "main-kernel time / peak")},
         "schedules": {k: {"ms": round(v["ms"], 5), "gflops": round(v["gflops"], 2),
                           "hbm_frac": round(v["hbm_frac"], 4)} for k, v in head.items()},
     }
 
     if not args.no_extra:
+        vend = vendor_leg(torch, pl, N, args.steps, stream)
+        ok = [v["gflops"] for v in vend.values() if "gflops" in v]
+        out["vendor_rocsparse"] = vend
+        out["vs_rocsparse_best"] = round(out["value"] / max(ok), 3) if ok else None
         forms = {}
         for dtype in ("f32", "bf16", "fp16"):
             res, _ = run_leg(torch, pl, 256, dtype, [("eb_sr", S.make_plan("eb_sr")),
@@ -259,6 +511,15 @@ def main():
         f32 = forms["powerlaw_n256_f32"]["gflops"]
         forms["bf16_vs_f32_n256"] = round(forms["powerlaw_n256_bf16"]["gflops"] / f32, 3)
         out["forms"] = forms
+
+    if not args.no_extra and args.batch_matrices > 0:
+        mats = cfg4_set(args.batch_matrices)
+        out["batched"] = {"f32": batch_leg(torch, mats, "f32", 5, 2, stream),
+                          "bf16": batch_leg(torch, mats, "bf16", 5, 2, stream)}
+        out["served"] = serve_leg(torch, mats, "bf16", args.serve_seconds, 2000.0, pl)
+        sat = out["served"]["powerlaw_n128_saturated"]["gflops"]
+        out["forms"]["served_bf16_powerlaw_vs_plain_f32"] = round(sat / out["value"], 3)
+        del mats
 
     if not args.no_cpu_baseline:
         log("timing CPU baseline (Eigen via oracle/_ref)")

@@ -69,6 +69,14 @@ void segtaco_clear_error(void);
  * (-1 before any call). */
 float get_gpu_timer_result(void);
 
+/* The operator surface as a table, for drivers and sweeps that iterate over
+ * it: segtaco_legacy_count() entry points, id -> name, and a call by id
+ * (equivalent to calling the named function). */
+int32_t segtaco_legacy_count(void);
+const char *segtaco_legacy_name(int32_t id);
+void segtaco_legacy_call(int32_t id, taco_tensor_t *C, taco_tensor_t *A,
+                         taco_tensor_t *B);
+
 /* The five north-star entry points. */
 /* spmm_rb_sr.h:220-243: row-balanced, sequential reduction. */
 void spmm_rb_sr(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
@@ -163,6 +171,14 @@ void spmm_eb_sr_256_16_1(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
 void spmm_eb_sr_256_16_2(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
 void spmm_eb_sr_256_16_4(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
 
+/* The vendor-library rows (spmm_csr_gpu_cusparse.h:52-170 and :171-260, the
+ * sweep's "alg2" row, spmm_csr_gpu.h:343): C = A * B with rocSPARSE's CSR
+ * SpMM (default algorithm, row-major B and C), the timer covering its compute
+ * stage only as the reference's does. A comparison row for sweeps; values
+ * must be present. */
+void spmm_csr_cusparse_row(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+void spmm_csr_gpu_cusparse(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
+
 /* ------------------------------------------------------------------------ */
 /* Device-resident API                                                       */
 /* ------------------------------------------------------------------------ */
@@ -216,6 +232,17 @@ int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                      const segtaco_plan_t *plan, void *workspace,
                      size_t workspace_bytes, void *stream);
 
+/* Runs segtaco_spmm `reps` times (synchronously) and returns the mean ms of
+ * its three stages in out_ms[0..2]: row partition, main kernel, carry fix-up
+ * (0 for stages a plan does not have), each timed with HIP events on
+ * `stream`. */
+int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,
+                                 int32_t b_dtype, int64_t ldb, int32_t N,
+                                 float *C, int64_t ldc,
+                                 const segtaco_plan_t *plan, void *workspace,
+                                 size_t workspace_bytes, void *stream,
+                                 int32_t reps, float *out_ms);
+
 /* K21 row partition (gpu_library.h:206-215, 236-240), bit-exact:
  * out[b] = taco_binarySearchBefore(pos, 0, M, b * values_per_block) for
  * b = 0 .. num_blocks (num_blocks + 1 entries), device pointers. */
@@ -240,6 +267,15 @@ const char *segtaco_version(void);
 /* GPUs visible to the HIP runtime libsegtaco is bound to; a negative value is
  * minus the HIP error code of hipGetDeviceCount. */
 int32_t segtaco_device_count(void);
+
+/* rocSPARSE CSR SpMM on device operands (FP32 values, row-major B and C):
+ * the in-box vendor library as a comparison for benchmarks, not an engine
+ * schedule. alg: 0 default, 1 csr, 2 csr_row_split, 3 csr_nnz_split,
+ * 4 csr_merge_path. With ms_out non-NULL the call is synchronous and
+ * *ms_out gets the compute stage's event time. */
+int32_t segtaco_vendor_spmm(const segtaco_csr_t *A, const float *B, int64_t ldb,
+                            int32_t N, float *C, int64_t ldc, int32_t alg,
+                            void *stream, float *ms_out);
 
 #ifdef __cplusplus
 }

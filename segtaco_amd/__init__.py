@@ -123,6 +123,14 @@ class TacoTensor(ctypes.Structure):
                 ("vals_size", ctypes.c_int32)]
 
 
+class MtxInfo(ctypes.Structure):
+    """segtaco_mtx_info_t."""
+    _fields_ = [("coordinate", ctypes.c_int32), ("field", ctypes.c_int32),
+                ("symmetry", ctypes.c_int32), ("entries", ctypes.c_int32)]
+
+
+MTX_REFERENCE_TRUNCATE = 1
+
 _LIB = None
 
 
@@ -164,6 +172,11 @@ def lib() -> ctypes.CDLL:
     L.segtaco_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32, ctypes.c_int64,
                                ctypes.c_int32, _vp, ctypes.c_int64, ctypes.POINTER(Plan), _vp,
                                ctypes.c_size_t, _vp]
+    L.segtaco_spmm_stage_times.restype = ctypes.c_int32
+    L.segtaco_spmm_stage_times.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32,
+                                           ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int64,
+                                           ctypes.POINTER(Plan), _vp, ctypes.c_size_t, _vp,
+                                           ctypes.c_int32, _f32p]
     L.segtaco_block_starts.restype = ctypes.c_int32
     L.segtaco_block_starts.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        _vp, _vp]
@@ -260,6 +273,28 @@ def lib() -> ctypes.CDLL:
     L.segtaco_server_plan.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Plan)]
     L.segtaco_server_destroy.restype = ctypes.c_int32
     L.segtaco_server_destroy.argtypes = [_vp]
+    _i32pp = ctypes.POINTER(_i32p)
+    L.segtaco_mtx_read.restype = ctypes.c_int32
+    L.segtaco_mtx_read.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, _i32p, _i32p, _i32pp,
+                                   _i32pp, ctypes.POINTER(MtxInfo)]
+    L.segtaco_mtx_read_csr.restype = ctypes.c_int32
+    L.segtaco_mtx_read_csr.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, _i32p,
+                                       _i32p, _i32p, _i32pp, _i32pp, ctypes.POINTER(_f32p)]
+    L.segtaco_fill_reference_vals.restype = None
+    L.segtaco_fill_reference_vals.argtypes = [ctypes.c_uint64, ctypes.c_int64, _f32p]
+    L.segtaco_coo_to_csr.restype = ctypes.c_int32
+    L.segtaco_coo_to_csr.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p,
+                                     _f32p, _i32p, _i32p, _f32p, _i32p]
+    L.segtaco_host_free.restype = None
+    L.segtaco_host_free.argtypes = [_vp]
+    for fn in ("segtaco_compare_array", "segtaco_compare_array_my"):
+        getattr(L, fn).restype = ctypes.c_float
+        getattr(L, fn).argtypes = [_f32p, _f32p, ctypes.c_size_t]
+    L.segtaco_compare_matrices.restype = ctypes.c_float
+    L.segtaco_compare_matrices.argtypes = [ctypes.POINTER(TacoTensor), ctypes.POINTER(TacoTensor)]
+    L.segtaco_vendor_spmm.restype = ctypes.c_int32
+    L.segtaco_vendor_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int64, ctypes.c_int32,
+                                      _vp, ctypes.c_int64, ctypes.c_int32, _vp, _f32p]
     _LIB = L
     return L
 
@@ -409,6 +444,102 @@ def legacy(name: str, pos, crd, vals, B: np.ndarray, K: int | None = None,
 
 
 # ---------------------------------------------------------------------------
+# Operand construction and comparators (include/segtaco/io.h)
+# ---------------------------------------------------------------------------
+
+def _take(ptr, n, dtype):
+    """Copy a library-allocated array into numpy and free it."""
+    L = lib()
+    out = np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dtype, copy=True) \
+        if n > 0 else np.zeros(0, dtype=dtype)
+    L.segtaco_host_free(ctypes.cast(ptr, _vp))
+    return out
+
+
+def mtx_read(path: str, flags: int = 0):
+    """MatrixMarket pattern -> (nrow, ncol, rows, cols, info), sorted 0-based
+    coordinates (read_mtx_file, ds.h:183-268)."""
+    L = lib()
+    m, n, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    R, C = _i32p(), _i32p()
+    info = MtxInfo()
+    check(L.segtaco_mtx_read(path.encode(), flags, ctypes.byref(m), ctypes.byref(n),
+                             ctypes.byref(k), ctypes.byref(R), ctypes.byref(C),
+                             ctypes.byref(info)))
+    rows = _take(R, k.value, np.int32)
+    cols = _take(C, k.value, np.int32)
+    return m.value, n.value, rows, cols, info
+
+
+def mtx_read_csr(path: str, flags: int = 0, seed: int = 0):
+    """new_get_matrix + to_eigen_csr: (M, K, pos, crd, vals). seed = 2**64 - 1
+    fills ones instead of the {0, 0.1, 0.2} reference values."""
+    L = lib()
+    M, K, nnz = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    P, D, V = _i32p(), _i32p(), _f32p()
+    check(L.segtaco_mtx_read_csr(path.encode(), flags, ctypes.c_uint64(seed), ctypes.byref(M),
+                                 ctypes.byref(K), ctypes.byref(nnz), ctypes.byref(P),
+                                 ctypes.byref(D), ctypes.byref(V)))
+    pos = _take(P, M.value + 1, np.int32)
+    return M.value, K.value, pos, _take(D, nnz.value, np.int32), _take(V, nnz.value, np.float32)
+
+
+def fill_reference_vals(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.float32)
+    lib().segtaco_fill_reference_vals(ctypes.c_uint64(seed), n, out.ctypes.data_as(_f32p))
+    return out
+
+
+def coo_to_csr(m: int, n: int, rows, cols, vals=None):
+    """COO -> compressed CSR (to_eigen_csr, ds.h:380-397): (pos, crd, vals)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    k = len(rows)
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.float32)
+    pos = np.empty(m + 1, dtype=np.int32)
+    crd = np.empty(max(k, 1), dtype=np.int32)
+    ov = np.empty(max(k, 1), dtype=np.float32)
+    out = ctypes.c_int32()
+    check(lib().segtaco_coo_to_csr(m, n, k, rows.ctypes.data_as(_i32p), cols.ctypes.data_as(_i32p),
+                                   v.ctypes.data_as(_f32p) if v is not None else None,
+                                   pos.ctypes.data_as(_i32p), crd.ctypes.data_as(_i32p),
+                                   ov.ctypes.data_as(_f32p), ctypes.byref(out)))
+    c = out.value
+    return pos, crd[:c].copy(), (ov[:c].copy() if v is not None else None)
+
+
+def compare_array(x, y) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+    return float(lib().segtaco_compare_array(x.ctypes.data_as(_f32p), y.ctypes.data_as(_f32p),
+                                             x.size))
+
+
+def compare_array_my(x, y) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+    return float(lib().segtaco_compare_array_my(x.ctypes.data_as(_f32p),
+                                                y.ctypes.data_as(_f32p), x.size))
+
+
+def compare_matrices(a: np.ndarray, b: np.ndarray) -> float:
+    keep: list = []
+    ta = _taco_dense(np.ascontiguousarray(a, dtype=np.float32), keep)
+    tb = _taco_dense(np.ascontiguousarray(b, dtype=np.float32), keep)
+    return float(lib().segtaco_compare_matrices(ctypes.byref(ta), ctypes.byref(tb)))
+
+
+def vendor_spmm(A: "CSR", B_ptr: int, ldb: int, N: int, C_ptr: int, ldc: int, alg: int = 0,
+                stream: int | None = None, timed: bool = False):
+    """rocSPARSE CSR SpMM on device operands (comparison row). Returns the
+    compute-stage ms when timed (synchronous), else None (asynchronous)."""
+    ms = ctypes.c_float(0.0)
+    check(lib().segtaco_vendor_spmm(ctypes.byref(A), B_ptr, ldb, N, C_ptr, ldc, alg,
+                                    stream or None, ctypes.byref(ms) if timed else None))
+    return float(ms.value) if timed else None
+
+
+# ---------------------------------------------------------------------------
 # Device-resident API (device pointers as integers)
 # ---------------------------------------------------------------------------
 
@@ -434,6 +565,18 @@ def spmm(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: in
     """C = A * B on device memory (asynchronous on `stream`)."""
     check(lib().segtaco_spmm(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
                              ctypes.byref(plan), ws_ptr or None, ws_bytes, stream or None))
+
+
+def spmm_stage_times(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: int,
+                     plan: Plan, ws_ptr: int | None, ws_bytes: int, stream: int | None = None,
+                     reps: int = 10) -> tuple[float, float, float]:
+    """Mean (partition, main kernel, fix-up) milliseconds over `reps` runs,
+    timed with HIP events on `stream` (synchronous)."""
+    out = (ctypes.c_float * 3)()
+    check(lib().segtaco_spmm_stage_times(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
+                                         ctypes.byref(plan), ws_ptr or None, ws_bytes,
+                                         stream or None, reps, out))
+    return float(out[0]), float(out[1]), float(out[2])
 
 
 def block_starts(pos_ptr: int, M: int, T: int, num_blocks: int, out_ptr: int,

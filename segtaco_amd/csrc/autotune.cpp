@@ -27,7 +27,7 @@ namespace segtaco {
 int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                   int64_t ldb, int32_t N, float *C, int64_t ldc,
                   const segtaco_plan_t *plan, void *ws, size_t ws_bytes,
-                  hipStream_t stream);
+                  hipStream_t stream, hipEvent_t *ev);
 size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
                                  const segtaco_plan_t *plan);
 int32_t plan_resolve_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
@@ -271,11 +271,11 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
   segtaco_plan_t best_plan = model;
   int32_t status = SEGTACO_OK;
   for (const auto &p : plans) {
-    status = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, &p, ws, ws_need, stream);
+    status = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, &p, ws, ws_need, stream, nullptr);
     if (status != SEGTACO_OK) break;
     if (hipEventRecord(e0, stream) != hipSuccess) { status = SEGTACO_ERR_HIP; break; }
     for (int32_t r = 0; r < reps && status == SEGTACO_OK; ++r)
-      status = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, &p, ws, ws_need, stream);
+      status = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, &p, ws, ws_need, stream, nullptr);
     if (status != SEGTACO_OK) break;
     if (hipEventRecord(e1, stream) != hipSuccess ||
         hipEventSynchronize(e1) != hipSuccess) {
@@ -297,7 +297,7 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
   // Leave C holding the winner's result.
   if (ws_need) SEGTACO_HIP(hipMalloc(&ws, ws_need));
   status = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, &best_plan, ws, ws_need,
-                     stream);
+                     stream, nullptr);
   (void)hipStreamSynchronize(stream);
   if (ws) (void)hipFree(ws);
   if (status != SEGTACO_OK) return status;

@@ -165,10 +165,13 @@ size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
   return eb_layout(A->nnz, N, tile).total;
 }
 
+// ev (optional, 4 events): recorded before the partition, after it, after
+// the main kernel and after the fix-up (row-balanced plans have neither
+// partition nor fix-up; their events bracket nothing).
 int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                   int64_t ldb, int32_t N, float *C, int64_t ldc,
                   const segtaco_plan_t *plan_in, void *ws, size_t ws_bytes,
-                  hipStream_t stream) {
+                  hipStream_t stream, hipEvent_t *ev) {
   SEGTACO_RET(check_csr(A));
   if (!plan_in) return set_error(SEGTACO_ERR_INVALID, "null plan");
   if (N < 0) return set_error(SEGTACO_ERR_INVALID, "N < 0");
@@ -203,18 +206,29 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   const int32_t V = plan.vec;
   a.panels = (int32_t)ceil_div(N, (int64_t)plan.col_lanes * V);
 
+  auto mark = [&](int i) -> hipError_t {
+    return ev ? hipEventRecord(ev[i], stream) : hipSuccess;
+  };
   switch (plan.schedule) {
     case SEGTACO_RB_SR: {
       const int64_t rows_per_block =
           (int64_t)kWavesPerBlock * (64 / plan.col_lanes) * plan.group;
       a.nblocks = (int32_t)ceil_div(A->M, rows_per_block);
+      SEGTACO_HIP(mark(0));
+      SEGTACO_HIP(mark(1));
       SEGTACO_HIP(launch_rb_sr(b_dtype, V, a, stream));
+      SEGTACO_HIP(mark(2));
+      SEGTACO_HIP(mark(3));
       return SEGTACO_OK;
     }
     case SEGTACO_RB_PR: {
       const int64_t rows_per_wave = 64 / (plan.col_lanes * plan.group);
       a.nblocks = (int32_t)ceil_div(A->M, rows_per_wave * kWavesPerBlock);
+      SEGTACO_HIP(mark(0));
+      SEGTACO_HIP(mark(1));
       SEGTACO_HIP(launch_rb_pr(b_dtype, V, a, stream));
+      SEGTACO_HIP(mark(2));
+      SEGTACO_HIP(mark(3));
       return SEGTACO_OK;
     }
     default:
@@ -233,9 +247,11 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   a.carry_head = reinterpret_cast<float *>(w + L.head_off);
   a.carry_tail = reinterpret_cast<float *>(w + L.tail_off);
   a.tile_row = reinterpret_cast<int32_t *>(w + L.rows_off);
+  SEGTACO_HIP(mark(0));
   SEGTACO_HIP(launch_block_starts(A->pos, A->M, plan.tile, L.ntiles,
                                   const_cast<int32_t *>(a.part),
                                   const_cast<int32_t *>(a.tile_row), stream));
+  SEGTACO_HIP(mark(1));
   if (plan.schedule == SEGTACO_EB_SR) {
     const int64_t tiles_per_block =
         (int64_t)kWavesPerBlock * (64 / plan.col_lanes);
@@ -245,7 +261,9 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     a.nblocks = (int32_t)ceil_div(L.ntiles, kWavesPerBlock);
     SEGTACO_HIP(launch_eb_pr(b_dtype, V, a, stream));
   }
+  SEGTACO_HIP(mark(2));
   SEGTACO_HIP(launch_fixup(V, a, stream));
+  SEGTACO_HIP(mark(3));
   return SEGTACO_OK;
 }
 
@@ -298,7 +316,40 @@ int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                      const segtaco_plan_t *plan, void *workspace,
                      size_t workspace_bytes, void *stream) {
   return spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace,
-                   workspace_bytes, static_cast<hipStream_t>(stream));
+                   workspace_bytes, static_cast<hipStream_t>(stream), nullptr);
+}
+
+int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,
+                                 int32_t b_dtype, int64_t ldb, int32_t N,
+                                 float *C, int64_t ldc,
+                                 const segtaco_plan_t *plan, void *workspace,
+                                 size_t workspace_bytes, void *stream_v,
+                                 int32_t reps, float *out_ms) {
+  if (!out_ms || reps < 1)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_spmm_stage_times: bad argument");
+  hipStream_t stream = static_cast<hipStream_t>(stream_v);
+  hipEvent_t ev[4] = {};
+  for (auto &e : ev) SEGTACO_HIP(hipEventCreate(&e));
+  double acc[3] = {0, 0, 0};
+  int32_t st = SEGTACO_OK;
+  for (int32_t r = 0; r < reps && st == SEGTACO_OK; ++r) {
+    st = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace,
+                   workspace_bytes, stream, ev);
+    if (st != SEGTACO_OK) break;
+    if (hipEventSynchronize(ev[3]) != hipSuccess) {
+      st = set_error(SEGTACO_ERR_HIP, "hipEventSynchronize");
+      break;
+    }
+    for (int i = 0; i < 3; ++i) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      acc[i] += ms;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (st != SEGTACO_OK) return st;
+  for (int i = 0; i < 3; ++i) out_ms[i] = (float)(acc[i] / reps);
+  return SEGTACO_OK;
 }
 
 int32_t segtaco_block_starts(const int32_t *pos, int32_t M,

@@ -24,11 +24,14 @@ namespace segtaco {
 int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                   int64_t ldb, int32_t N, float *C, int64_t ldc,
                   const segtaco_plan_t *plan, void *ws, size_t ws_bytes,
-                  hipStream_t stream);
+                  hipStream_t stream, hipEvent_t *ev);
 size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
                                  const segtaco_plan_t *plan);
 int32_t plan_resolve_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
                           int64_t ldb, int64_t ldc, segtaco_plan_t *plan);
+int32_t vendor_spmm(const segtaco_csr_t *A, const float *B, int64_t ldb,
+                    int32_t N, float *C, int64_t ldc, int32_t alg,
+                    hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
 
 namespace {
 
@@ -120,6 +123,7 @@ int32_t run_legacy(int id, taco_tensor_t *C, taco_tensor_t *A,
     return set_error(SEGTACO_ERR_INVALID, "%s: null vals", name);
 
   const Preset &pr = kPresets[id];
+  const bool vendor = pr.schedule == SEGTACO_VENDOR_ROW;
   segtaco_plan_t plan{};
   plan.schedule = pr.schedule;
   plan.col_lanes = pr.col_lanes;
@@ -133,7 +137,9 @@ int32_t run_legacy(int id, taco_tensor_t *C, taco_tensor_t *A,
     plan.vec = v;
   }
   segtaco_csr_t dA{M, K, nnz, SEGTACO_DT_F32, nullptr, nullptr, nullptr};
-  SEGTACO_RET(plan_resolve_impl(&dA, N, SEGTACO_DT_F32, N, N, &plan));
+  if (!vendor) SEGTACO_RET(plan_resolve_impl(&dA, N, SEGTACO_DT_F32, N, N, &plan));
+  if (vendor && nnz > 0 && !A->vals)
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "%s: the vendor row needs values", name);
 
   LegacyState &s = state();
   std::lock_guard<std::mutex> lock(s.mu);
@@ -143,7 +149,7 @@ int32_t run_legacy(int id, taco_tensor_t *C, taco_tensor_t *A,
   const size_t b_val = A->vals ? sizeof(float) * (size_t)nnz : 0;
   const size_t b_B = sizeof(float) * (size_t)K * N;
   const size_t b_C = sizeof(float) * (size_t)M * N;
-  const size_t b_ws = spmm_workspace_bytes_impl(&dA, N, &plan);
+  const size_t b_ws = vendor ? 0 : spmm_workspace_bytes_impl(&dA, N, &plan);
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -165,11 +171,18 @@ int32_t run_legacy(int id, taco_tensor_t *C, taco_tensor_t *A,
   dA.crd = reinterpret_cast<const int32_t *>(base + o_crd);
   dA.vals = b_val ? base + o_val : nullptr;
 
-  SEGTACO_HIP(hipEventRecord(s.ev0, s.stream));
-  SEGTACO_RET(spmm_impl(&dA, base + o_B, SEGTACO_DT_F32, N, N,
-                        reinterpret_cast<float *>(base + o_C), N, &plan,
-                        base + o_ws, b_ws, s.stream));
-  SEGTACO_HIP(hipEventRecord(s.ev1, s.stream));
+  if (vendor) {
+    // the vendor row times rocSPARSE's compute stage only
+    SEGTACO_RET(vendor_spmm(&dA, reinterpret_cast<const float *>(base + o_B), N, N,
+                            reinterpret_cast<float *>(base + o_C), N, 0, s.stream,
+                            s.ev0, s.ev1));
+  } else {
+    SEGTACO_HIP(hipEventRecord(s.ev0, s.stream));
+    SEGTACO_RET(spmm_impl(&dA, base + o_B, SEGTACO_DT_F32, N, N,
+                          reinterpret_cast<float *>(base + o_C), N, &plan,
+                          base + o_ws, b_ws, s.stream, nullptr));
+    SEGTACO_HIP(hipEventRecord(s.ev1, s.stream));
+  }
   SEGTACO_HIP(hipEventSynchronize(s.ev1));
   float ms = -1.0f;
   SEGTACO_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
