@@ -92,6 +92,9 @@ def ref_lib():
         R.ref_read_mtx.restype = ctypes.c_int32
         R.ref_read_mtx.argtypes = [ctypes.c_char_p, _i32p, _i32p, ctypes.POINTER(_i32p),
                                    ctypes.POINTER(_i32p)]
+        for fn in ("ref_compare_array", "ref_compare_array_my"):
+            getattr(R, fn).restype = ctypes.c_float
+            getattr(R, fn).argtypes = [_f32p, _f32p, ctypes.c_size_t]
         R.ref_free.restype = None
         R.ref_free.argtypes = [ctypes.c_void_p]
         R.ref_max_threads.restype = ctypes.c_int32
@@ -293,3 +296,24 @@ def ref_read_mtx(path: str):
     R.ref_free(ctypes.cast(p_pos, ctypes.c_void_p))
     R.ref_free(ctypes.cast(p_crd, ctypes.c_void_p))
     return nr.value, nc.value, pos, crd
+
+
+def ref_compare_array(x, y) -> float:
+    """ds.h:463-477 compare_array, compiled from the reference."""
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libsegtaco_ref.so not built")
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+    return float(R.ref_compare_array(_ptr(x, _f32p), _ptr(y, _f32p), x.size))
+
+
+def ref_compare_array_my(x, y) -> float:
+    """ds.h:478-492 compare_array_my, compiled from the reference (prints
+    each mismatch to stdout)."""
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libsegtaco_ref.so not built")
+    x = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+    return float(R.ref_compare_array_my(_ptr(x, _f32p), _ptr(y, _f32p), x.size))

@@ -97,6 +97,15 @@ int32_t ref_read_mtx(const char *path, int32_t *nrow, int32_t *ncol,
   return (int32_t)indices.size();
 }
 
+// The reference's result comparators, ds.h:463-499 (compare_array_my prints
+// every mismatch to stdout, as the reference does).
+float ref_compare_array(const float *x, const float *y, size_t n) {
+  return compare_array(x, y, n);
+}
+float ref_compare_array_my(const float *x, const float *y, size_t n) {
+  return compare_array_my(x, y, n);
+}
+
 void ref_free(void *p) { std::free(p); }
 
 int32_t ref_max_threads(void) {

@@ -34,8 +34,10 @@ COMMON = [
 
 LIB_SOURCES = [
     "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
-    "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp",
+    "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp", "io.cpp", "vendor.cpp",
 ]
+# rocSPARSE backs the vendor comparison row only (vendor.cpp).
+LIB_LINK = ["-L/opt/rocm/lib", "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _compile(src: str) -> str:
@@ -59,7 +61,7 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, LIB_SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + LIB_LINK
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

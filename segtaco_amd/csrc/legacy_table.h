@@ -19,7 +19,12 @@
 // spmm_rb_sr.h:363-376, spmm_csr_gpu_taco.h:133-134) every preset is correct
 // for any N.
 //
+// The two vendor-library rows (spmm_csr_gpu_cusparse.h:52-260) run
+// rocSPARSE's CSR SpMM instead of an engine schedule (SEGTACO_VENDOR_ROW):
+// a comparison row for sweeps, not an engine dispatch target.
+//
 // X(name, schedule, vec_cap, col_lanes, group, tile)
+#define SEGTACO_VENDOR_ROW 100
 #define SEGTACO_LEGACY_TABLE(X)                                   \
   X(spmm_rb_sr, SEGTACO_RB_SR, 0, 0, 0, 0)                        \
   X(spmm_rb_pr, SEGTACO_RB_PR, 0, 0, 0, 0)                        \
@@ -97,7 +102,9 @@
   X(spmm_eb_sr_256_8_4, SEGTACO_EB_SR, 4, 0, 0, 512)              \
   X(spmm_eb_sr_256_16_1, SEGTACO_EB_SR, 1, 0, 0, 1024)            \
   X(spmm_eb_sr_256_16_2, SEGTACO_EB_SR, 2, 0, 0, 1024)            \
-  X(spmm_eb_sr_256_16_4, SEGTACO_EB_SR, 4, 0, 0, 1024)
+  X(spmm_eb_sr_256_16_4, SEGTACO_EB_SR, 4, 0, 0, 1024)            \
+  X(spmm_csr_cusparse_row, SEGTACO_VENDOR_ROW, 0, 0, 0, 0)        \
+  X(spmm_csr_gpu_cusparse, SEGTACO_VENDOR_ROW, 0, 0, 0, 0)
 
 enum segtaco_legacy_id {
 #define SEGTACO_LEGACY_ENUM(name, s, v, q, g, t) SEGTACO_LEGACY_##name,
