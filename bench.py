@@ -128,6 +128,41 @@ def main_kernel_ms(torch, ops, N, plans, name, reps, stream):
     return st
 
 
+def sddmm_leg(torch, ops, J, dtype, steps, warmup, stream):
+    """SDDMM on the headline pattern: out[p] = s_p * dot(X[row(p)], Y[crd[p]])
+    with X, Y of width J; compulsory bytes 4(M+1) + nnz(4 + s) + K_touched*J*s
+    + M*J*s + 4*nnz (X and Y rows read once, out written once)."""
+    s_e = ELEM[dtype]
+    A = ops.desc(dtype)
+    X = ops.dense(31, J, dtype)  # K = M here, so one generator shape fits both
+    Y = ops.dense(32, J, dtype)
+    out = torch.empty(ops.nnz, dtype=torch.float32, device="cuda")
+    ws_bytes = S.sddmm_workspace_bytes(A)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    args = (A, X.data_ptr(), J, Y.data_ptr(), J, J, S.DTYPES[dtype], out.data_ptr(),
+            ws.data_ptr(), ws_bytes, stream.cuda_stream)
+    for _ in range(warmup):
+        S.sddmm(*args)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        S.sddmm(*args)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    cbytes = 4 * (ops.M + 1) + ops.nnz * (4 + s_e) + ops.k_touched * J * s_e + \
+        ops.M * J * s_e + 4 * ops.nnz
+    res = {"J": J, "dtype": dtype, "ms": round(ms, 5),
+           "gflops": round(2.0 * ops.nnz * J / (ms * 1e-3) / 1e9, 2),
+           "compulsory_bytes": cbytes,
+           "hbm_frac": round(cbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"SDDMM J={J} {dtype}: {ms*1e3:.1f} us  {res['gflops']:.1f} GFLOP/s")
+    del X, Y, out, ws
+    torch.cuda.empty_cache()
+    return res
+
+
 VENDOR_ALGS = {0: "default", 1: "csr", 2: "csr_row_split", 3: "csr_nnz_split",
                4: "csr_merge_path"}
 
@@ -493,6 +528,8 @@ def main():
         ok = [v["gflops"] for v in vend.values() if "gflops" in v]
         out["vendor_rocsparse"] = vend
         out["vs_rocsparse_best"] = round(out["value"] / max(ok), 3) if ok else None
+        out["sddmm"] = {d: sddmm_leg(torch, pl, 128, d, args.steps, args.warmup, stream)
+                        for d in ("f32", "bf16")}
         forms = {}
         for dtype in ("f32", "bf16", "fp16"):
             res, _ = run_leg(torch, pl, 256, dtype, [("eb_sr", S.make_plan("eb_sr")),

@@ -63,6 +63,9 @@ def lib():
         L.or_spmm_absbound.restype = None
         L.or_spmm_absbound.argtypes = [ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f32p,
                                        _f32p, ctypes.c_int64, _f64p, ctypes.c_int64]
+        L.or_sddmm_f64.restype = None
+        L.or_sddmm_f64.argtypes = [ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f32p, _f32p,
+                                   ctypes.c_int64, _f32p, ctypes.c_int64, ctypes.c_int32, _f64p]
         L.or_max_scaled_err.restype = ctypes.c_double
         L.or_max_scaled_err.argtypes = [ctypes.c_int64, _f32p, _f64p, _f64p]
         L.or_round_array.restype = None
@@ -175,6 +178,26 @@ def spmm_f64(pos, crd, vals, B) -> np.ndarray:
     lib().or_spmm_f64(M, N, _ptr(pos, _i32p), _ptr(crd, _i32p), _ptr(vals, _f32p),
                       _ptr(B, _f32p), N, _ptr(C, _f64p), N)
     return C
+
+
+def sddmm_f64(pos, crd, vals, X, Y, absval: bool = False) -> np.ndarray:
+    """SDDMM in double: out[p] = s_p * dot(X[row(p)], Y[crd[p]]); with absval,
+    the error scale |s_p| * sum |X||Y|."""
+    M = len(pos) - 1
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    crd = np.ascontiguousarray(crd, dtype=np.int32)
+    if crd.size == 0:
+        crd = np.zeros(1, dtype=np.int32)
+    vals = None if vals is None else np.ascontiguousarray(vals, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    J = X.shape[1]
+    nnz = int(pos[-1]) if M > 0 else 0
+    out = np.empty(max(nnz, 1), dtype=np.float64)
+    lib().or_sddmm_f64(M, J, _ptr(pos, _i32p), _ptr(crd, _i32p), _ptr(vals, _f32p),
+                       _ptr(X, _f32p), J, _ptr(Y, _f32p), Y.shape[1], int(absval),
+                       _ptr(out, _f64p))
+    return out[:nnz]
 
 
 def spmm_absbound(pos, crd, vals, B) -> np.ndarray:

@@ -123,6 +123,27 @@ void or_spmm_absbound(int32_t M, int32_t N, const int32_t *pos,
   }
 }
 
+/* SDDMM in double (the reference's sddmm_csr_gpu_taco.h:70-148 computes
+ * A_vals[p] = sum_j B_vals[p] * C[i, j] * D[k, j] in FP64): out[p] =
+ * s_p * sum_j X[i, j] * Y[k, j] for stored entry p = (i, k); with absval set,
+ * |s_p| * sum_j |X[i, j]| * |Y[k, j]| (the rounding-error scale) instead. */
+void or_sddmm_f64(int32_t M, int32_t J, const int32_t *pos, const int32_t *crd,
+                  const float *vals, const float *X, int64_t ldx,
+                  const float *Y, int64_t ldy, int32_t absval, double *out) {
+  for (int32_t i = 0; i < M; ++i) {
+    const float *x = X + (int64_t)i * ldx;
+    for (int32_t p = pos[i]; p < pos[i + 1]; ++p) {
+      const float *y = Y + (int64_t)crd[p] * ldy;
+      double d = 0.0;
+      for (int32_t j = 0; j < J; ++j)
+        d += absval ? fabs((double)x[j]) * fabs((double)y[j])
+                    : (double)x[j] * (double)y[j];
+      const double v = vals ? (double)vals[p] : 1.0;
+      out[p] = (absval ? fabs(v) : v) * d;
+    }
+  }
+}
+
 /* max_i |C_i - ref_i| / (scale_i + 1e-30); NaN anywhere returns +inf. */
 double or_max_scaled_err(int64_t n, const float *C, const double *ref,
                          const double *scale) {

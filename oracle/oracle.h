@@ -19,6 +19,10 @@
  *                         by special cases (A = I, B = I, B = ones, N = 1) and a
  *                         dense brute-force product.
  *   or_spmm_absbound      sum_p |v_p| * |B[crd_p, k]| in double (error scale).
+ *   or_sddmm_f64          SDDMM closed form in double (sddmm_csr_gpu_taco.h:
+ *                         70-148 computes it in FP64); pinned by a dense
+ *                         brute-force (X Y^T masked by the pattern) and special
+ *                         cases (J = 1, X = Y rows -> squared norms).
  *   or_bf16_round / or_f16_round  IEEE round-to-nearest-even narrowing; pinned
  *                         against numpy / torch conversions in tests.
  */
@@ -48,6 +52,9 @@ void or_spmm_f64(int32_t M, int32_t N, const int32_t *pos, const int32_t *crd,
 void or_spmm_absbound(int32_t M, int32_t N, const int32_t *pos,
                       const int32_t *crd, const float *vals, const float *B,
                       int64_t ldb, double *S, int64_t lds);
+void or_sddmm_f64(int32_t M, int32_t J, const int32_t *pos, const int32_t *crd,
+                  const float *vals, const float *X, int64_t ldx,
+                  const float *Y, int64_t ldy, int32_t absval, double *out);
 double or_max_scaled_err(int64_t n, const float *C, const double *ref,
                          const double *scale);
 

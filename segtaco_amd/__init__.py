@@ -292,6 +292,11 @@ def lib() -> ctypes.CDLL:
         getattr(L, fn).argtypes = [_f32p, _f32p, ctypes.c_size_t]
     L.segtaco_compare_matrices.restype = ctypes.c_float
     L.segtaco_compare_matrices.argtypes = [ctypes.POINTER(TacoTensor), ctypes.POINTER(TacoTensor)]
+    L.segtaco_sddmm_workspace_bytes.restype = ctypes.c_size_t
+    L.segtaco_sddmm_workspace_bytes.argtypes = [ctypes.POINTER(CSR)]
+    L.segtaco_sddmm.restype = ctypes.c_int32
+    L.segtaco_sddmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int64, _vp, ctypes.c_int64,
+                                ctypes.c_int32, ctypes.c_int32, _vp, _vp, ctypes.c_size_t, _vp]
     L.segtaco_vendor_spmm.restype = ctypes.c_int32
     L.segtaco_vendor_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int64, ctypes.c_int32,
                                       _vp, ctypes.c_int64, ctypes.c_int32, _vp, _f32p]
@@ -577,6 +582,17 @@ def spmm_stage_times(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: 
                                          ctypes.byref(plan), ws_ptr or None, ws_bytes,
                                          stream or None, reps, out))
     return float(out[0]), float(out[1]), float(out[2])
+
+
+def sddmm_workspace_bytes(S: CSR) -> int:
+    return int(lib().segtaco_sddmm_workspace_bytes(ctypes.byref(S)))
+
+
+def sddmm(S: CSR, X_ptr: int, ldx: int, Y_ptr: int, ldy: int, J: int, dtype: int, out_ptr: int,
+          ws_ptr: int | None, ws_bytes: int, stream: int | None = None) -> None:
+    """out[p] = s_p * dot(X[row(p)], Y[crd[p]]) on device memory (asynchronous)."""
+    check(lib().segtaco_sddmm(ctypes.byref(S), X_ptr, ldx, Y_ptr, ldy, J, dtype, out_ptr,
+                              ws_ptr or None, ws_bytes, stream or None))
 
 
 def block_starts(pos_ptr: int, M: int, T: int, num_blocks: int, out_ptr: int,
