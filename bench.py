@@ -441,7 +441,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=25)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--schedules", default="rb_sr,rb_pr,eb_sr,eb_pr")
+    ap.add_argument("--schedules", default="rb_sr,rb_pr,eb_sr,eb_pr,eb_cs")
     ap.add_argument("--plans", default="",
                     help="';'-separated plans for the headline leg instead of --schedules, "
                          "each 'schedule[:key=value[,key=value]]', e.g. 'eb_sr:tile=256'")

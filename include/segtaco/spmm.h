@@ -190,6 +190,11 @@ void spmm_csr_gpu_cusparse(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B)
 #define SEGTACO_RB_PR 1 /* row-balanced, parallel (tree) reduction */
 #define SEGTACO_EB_SR 2 /* element-balanced, sequential reduction */
 #define SEGTACO_EB_PR 3 /* element-balanced, segmented parallel reduction */
+/* eb_sr's tiles, each swept in ascending column order so that all of the
+ * device's lane groups gather from a narrow band of B at a time (L2 reuse);
+ * the rows of a unit of tiles accumulate in LDS. Same results as eb_sr with
+ * the same tile for column-sorted rows; no compensated summation. */
+#define SEGTACO_EB_CS 4
 
 /* A schedule and its parameters. Zero fields are resolved by
  * segtaco_plan_resolve from the operands. */

@@ -184,6 +184,11 @@ int32_t segtaco_batch_create(const segtaco_batch_item_t *items, int32_t n,
       b->fix_prefix[i + 1] = b->fix_prefix[i];
       continue;
     }
+    if (plan.schedule == SEGTACO_EB_CS) {
+      // its unit tables and LDS sizing assume one standalone launch
+      st = set_error(SEGTACO_ERR_UNSUPPORTED, "batch item %d: eb_cs runs standalone only", i);
+      break;
+    }
     // Alignment-aware resolution (vec from the real pointers).
     {
       segtaco_csr_t Ar = A;

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "cs.h"
 #include "launch.h"
 
 namespace segtaco {
@@ -43,6 +44,75 @@ EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile) {
   return L;
 }
 
+// Workspace of an eb_cs plan: the eb layout (partition, tile rows, carries)
+// followed by the sorted copy of A and the unit tables k_cs_prep writes.
+struct CsLayout {
+  EbLayout eb;
+  int32_t units, cap;
+  size_t col_off, val_off, slot_off, rows_off, nslot_off, count_off, flag_off, total;
+};
+
+int32_t device_cus();
+size_t device_max_lds();
+
+CsLayout cs_layout(int32_t nnz, int32_t N, int32_t tile, int32_t group,
+                   int32_t vec, int32_t elem, bool has_vals) {
+  CsLayout L{};
+  L.eb = eb_layout(nnz, N, tile);
+  L.units = (int32_t)ceil_div(L.eb.ntiles, group);
+  const size_t budget = device_max_lds() < kCsLdsBudget ? device_max_lds() : kCsLdsBudget;
+  L.cap = (int32_t)(budget / ((size_t)kCsLanes * vec * sizeof(float)));
+  size_t off = L.eb.total;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  L.col_off = take(sizeof(int32_t) * (size_t)nnz);
+  L.val_off = take(has_vals ? (size_t)elem * nnz : 0);
+  L.slot_off = take(sizeof(uint16_t) * (size_t)nnz);
+  L.rows_off = take(sizeof(int32_t) * (size_t)L.units * L.cap);
+  L.nslot_off = take(sizeof(int32_t) * (size_t)L.units);
+  L.count_off = take(sizeof(int32_t) * (size_t)L.eb.ntiles);
+  L.flag_off = take((size_t)L.eb.ntiles);
+  L.total = off;
+  return L;
+}
+
+// LDS bytes one workgroup may use on the current device (160 KiB on gfx950;
+// also the answer when no device is visible).
+size_t device_max_lds() {
+  static size_t cached = 0;
+  if (cached > 0) return cached;
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+          hipSuccess &&
+      n > 0)
+    cached = (size_t)n;
+  else
+    cached = 160 * 1024;
+  (void)hipGetLastError();
+  return cached;
+}
+
+// Compute units of the current device (256 on MI355X; also the answer when
+// no device is visible, so plans resolve the same on a host without a GPU).
+int32_t device_cus() {
+  static int32_t cached = 0;
+  if (cached > 0) return cached;
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
+          hipSuccess &&
+      n > 0)
+    cached = n;
+  else
+    cached = 256;
+  (void)hipGetLastError();
+  return cached;
+}
+
 namespace {
 
 bool aligned(const void *p, size_t a) {
@@ -72,7 +142,7 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
                 int64_t ldb, int64_t ldc, const void *B, const void *C,
                 segtaco_plan_t *plan) {
   if (!A || !plan) return set_error(SEGTACO_ERR_INVALID, "null argument");
-  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_EB_PR)
+  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_EB_CS)
     return set_error(SEGTACO_ERR_INVALID, "unknown schedule %d",
                      plan->schedule);
   if (b_dtype < SEGTACO_DT_F32 || b_dtype > SEGTACO_DT_F16)
@@ -85,6 +155,8 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
   const int32_t maxv = b_dtype == SEGTACO_DT_F32 ? 4 : 8;
   if (plan->vec == 0) {
     plan->vec = widest_vec(b_dtype, N, ldb, ldc, B, C);
+    // eb_cs keeps W = 8 * vec floats per row in LDS: at most 4 per lane
+    if (plan->schedule == SEGTACO_EB_CS && plan->vec > 4) plan->vec = 4;
   } else {
     const int32_t v = plan->vec;
     if (!is_pow2(v) || v > maxv)
@@ -123,6 +195,32 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
       if (plan->group == 0) plan->group = 64 / plan->col_lanes;
       if (plan->tile == 0) plan->tile = 256;
       break;
+    case SEGTACO_EB_CS: {
+      if (plan->col_lanes == 0) plan->col_lanes = kCsLanes;
+      if (plan->group == 0) plan->group = kCsMaxGroup;
+      if (plan->tile == 0) {
+        // one unit per CU: nnz / (CUs * tiles per unit), a multiple of 8
+        const int64_t per = ceil_div((int64_t)A->nnz, (int64_t)device_cus() * plan->group);
+        int64_t t = ceil_div(per, 8) * 8;
+        if (t < 32) t = 32;
+        if (t > kCsMaxTile) t = kCsMaxTile;
+        plan->tile = (int32_t)t;
+      }
+      if (plan->col_lanes != kCsLanes)
+        return set_error(SEGTACO_ERR_INVALID, "eb_cs: col_lanes must be %d", kCsLanes);
+      if (plan->group > kCsMaxGroup)
+        return set_error(SEGTACO_ERR_INVALID, "eb_cs: group %d > %d", plan->group,
+                         kCsMaxGroup);
+      if (plan->vec > 4)
+        return set_error(SEGTACO_ERR_UNSUPPORTED, "eb_cs: vec %d > 4", plan->vec);
+      if (plan->tile > kCsMaxTile)
+        return set_error(SEGTACO_ERR_INVALID, "eb_cs: tile %d > %d", plan->tile, kCsMaxTile);
+      if (plan->comp_threshold == -1) plan->comp_threshold = 0;
+      if (plan->comp_threshold != 0)
+        return set_error(SEGTACO_ERR_UNSUPPORTED,
+                         "eb_cs: compensated summation is not supported");
+      break;
+    }
   }
   const int32_t Q = plan->col_lanes, G = plan->group;
   if (!is_pow2(Q) || Q > 64)
@@ -159,6 +257,12 @@ int32_t check_csr(const segtaco_csr_t *A) {
 size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
                                  const segtaco_plan_t *plan) {
   if (!A || !plan || A->nnz == 0 || A->M == 0) return 0;
+  if (plan->schedule == SEGTACO_EB_CS) {
+    if (!plan->tile || !plan->group || !plan->vec) return 0;  // unresolved
+    return cs_layout(A->nnz, N, plan->tile, plan->group, plan->vec,
+                     elem_bytes(A->vals_dtype), A->vals != nullptr)
+        .total;
+  }
   if (plan->schedule != SEGTACO_EB_SR && plan->schedule != SEGTACO_EB_PR)
     return 0;
   const int32_t tile = plan->tile ? plan->tile : 128;
@@ -233,6 +337,65 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     }
     default:
       break;
+  }
+  if (plan.schedule == SEGTACO_EB_CS) {
+    const CsLayout L = cs_layout(A->nnz, N, plan.tile, plan.group, V,
+                                 elem_bytes(b_dtype), A->vals != nullptr);
+    if (!ws || ws_bytes < L.total)
+      return set_error(SEGTACO_ERR_INVALID,
+                       "workspace of %zu bytes needed, %zu given", L.total,
+                       ws ? ws_bytes : (size_t)0);
+    char *w = static_cast<char *>(ws);
+    a.ntiles = L.eb.ntiles;
+    a.rows_per_slice = (int32_t)ceil_div(A->M, L.eb.ntiles);
+    a.part = reinterpret_cast<int32_t *>(w + L.eb.part_off);
+    a.carry_head = reinterpret_cast<float *>(w + L.eb.head_off);
+    a.carry_tail = reinterpret_cast<float *>(w + L.eb.tail_off);
+    a.tile_row = reinterpret_cast<int32_t *>(w + L.eb.rows_off);
+    CsArgs c{};
+    c.M = A->M;
+    c.K = A->K;
+    c.N = N;
+    c.nnz = A->nnz;
+    c.pos = A->pos;
+    c.crd = A->crd;
+    c.vals = A->vals;
+    c.B = B;
+    c.ldb = ldb;
+    c.C = C;
+    c.ldc = ldc;
+    c.tile = plan.tile;
+    c.ntiles = L.eb.ntiles;
+    c.group = plan.group;
+    c.units = L.units;
+    const int32_t cus = device_cus();
+    c.rounds = (int32_t)ceil_div(L.units, cus);
+    c.cap = L.cap;
+    c.elem = elem_bytes(b_dtype);
+    c.tile_row = a.tile_row;
+    c.carry_head = a.carry_head;
+    c.carry_tail = a.carry_tail;
+    c.s_col = reinterpret_cast<int32_t *>(w + L.col_off);
+    c.s_val = A->vals ? static_cast<void *>(w + L.val_off) : nullptr;
+    c.s_slot = reinterpret_cast<uint16_t *>(w + L.slot_off);
+    c.slot_rows = reinterpret_cast<int32_t *>(w + L.rows_off);
+    c.n_slots = reinterpret_cast<int32_t *>(w + L.nslot_off);
+    c.tile_count = reinterpret_cast<int32_t *>(w + L.count_off);
+    c.tile_flags = reinterpret_cast<uint8_t *>(w + L.flag_off);
+    // the fix-up runs with its own natural lane count for N
+    a.lq = ilog2(natural_lanes(N, V));
+    a.panels = (int32_t)ceil_div(N, (int64_t)(1 << a.lq) * V);
+    SEGTACO_HIP(mark(0));
+    SEGTACO_HIP(launch_block_starts(A->pos, A->M, plan.tile, L.eb.ntiles,
+                                    const_cast<int32_t *>(a.part),
+                                    const_cast<int32_t *>(a.tile_row), stream));
+    SEGTACO_HIP(launch_cs_prep(c, stream));
+    SEGTACO_HIP(mark(1));
+    SEGTACO_HIP(launch_eb_cs(b_dtype, V, c, stream));
+    SEGTACO_HIP(mark(2));
+    SEGTACO_HIP(launch_fixup(V, a, stream));
+    SEGTACO_HIP(mark(3));
+    return SEGTACO_OK;
   }
   // Element-balanced: partition, tiles, carry fix-up.
   const EbLayout L = eb_layout(A->nnz, N, plan.tile);

@@ -151,3 +151,28 @@ def test_generators_deterministic_and_shaped():
     assert np.all(np.diff(pu) == 16)
     d1 = S.gen_dense(3, 100, 7)
     assert np.array_equal(d1, S.gen_dense(3, 100, 7)) and abs(d1.mean()) < 0.1
+
+
+def test_eb_cs_plan_resolution():
+    A = _desc(M=262144, K=262144, nnz=4204209)
+    p = S.plan_resolve(A, 128, S.DT_F32, S.make_plan("eb_cs"))
+    # one 64-tile unit per compute unit (256 on MI355X): 4204209 / 16384 -> 264
+    assert (p.vec, p.col_lanes, p.group, p.tile, p.comp_threshold) == (4, 8, 64, 264, 0)
+    assert S.workspace_bytes(A, 128, p) > S.workspace_bytes(A, 128, S.plan_resolve(
+        A, 128, S.DT_F32, S.make_plan("eb_sr", tile=264)))
+    # 16-bit storage: vector capped at 4; eb_cs never compensates (the
+    # default resolves to 0, an explicit threshold is refused)
+    B16 = _desc(M=262144, K=262144, nnz=4204209, dtype=S.DT_BF16)
+    p = S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("eb_cs"), 256, 256)
+    assert p.vec == 4 and p.col_lanes == 8 and p.comp_threshold == 0
+    with pytest.raises(S.SegtacoError):
+        S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("eb_cs", comp_threshold=1024), 256, 256)
+    with pytest.raises(S.SegtacoError):
+        S.plan_resolve(A, 128, S.DT_F32, S.make_plan("eb_cs", col_lanes=16))
+    with pytest.raises(S.SegtacoError):
+        S.plan_resolve(A, 128, S.DT_F32, S.make_plan("eb_cs", tile=1024))
+    with pytest.raises(S.SegtacoError):
+        S.plan_resolve(A, 128, S.DT_F32, S.make_plan("eb_cs", group=128))
+    # small matrices: the tile floor
+    p = S.plan_resolve(_desc(M=100, K=80, nnz=1000), 64, S.DT_F32, S.make_plan("eb_cs"))
+    assert p.tile == 32

@@ -1,4 +1,4 @@
-"""GPU parity of the four schedules against the CPU oracle, through the C-ABI.
+"""GPU parity of the five schedules against the CPU oracle, through the C-ABI.
 
 Tolerances (stated on the metric the oracle defines, oracle.scaled_err:
 max |C - ref| / sum_p |v_p| |B[crd_p, k]|):
@@ -30,6 +30,8 @@ PLANS = [
     ("eb_sr", {}), ("eb_sr", dict(tile=8)), ("eb_sr", dict(tile=33)), ("eb_sr", dict(tile=1000)),
     ("eb_pr", {}), ("eb_pr", dict(group=32, tile=64)), ("eb_pr", dict(group=4, tile=512)),
     ("eb_pr", dict(group=64, tile=8)),
+    ("eb_cs", {}), ("eb_cs", dict(tile=32)), ("eb_cs", dict(tile=512, group=8)),
+    ("eb_cs", dict(vec=1, tile=40)),
 ]
 PLAN_IDS = [f"{s}-" + "-".join(f"{k}{v}" for k, v in kw.items()) for s, kw in PLANS]
 
