@@ -45,6 +45,26 @@ __global__ __launch_bounds__(kBlockThreads) void k_block_starts(
   }
 }
 
+// rows[b] = the row holding entry b*T (the repaired K21 row), by scatter:
+// non-empty row r holds entries [pos[r], pos[r+1]), i.e. the tile starts
+// b*T for b in [ceil(pos[r]/T), floor((pos[r+1]-1)/T)]; rows[ntiles] = M.
+// One coalesced pass over pos instead of a bisection per tile.
+__global__ __launch_bounds__(kBlockThreads) void k_tile_rows(
+    const int32_t *__restrict__ pos, int32_t M, int32_t T, int32_t ntiles,
+    int32_t *__restrict__ rows) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < M) {
+    const int32_t p0 = pos[r], p1 = pos[r + 1];
+    if (p0 < p1) {
+      const int32_t b1 = (p1 - 1) / T;
+      for (int32_t b = (int32_t)(((int64_t)p0 + T - 1) / T); b <= b1; ++b)
+        rows[b] = (int32_t)r;
+    }
+  } else if (r == M) {
+    rows[ntiles] = M;
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_fill_zero(float *C,
                                                              int32_t M,
                                                              int32_t N,
@@ -120,6 +140,15 @@ hipError_t launch_block_starts(const int32_t *pos, int32_t M, int32_t T,
   const int64_t grid = (n + kBlockThreads - 1) / kBlockThreads;
   hipLaunchKernelGGL(k_block_starts, dim3((unsigned)grid), dim3(kBlockThreads),
                      0, s, pos, M, T, num_blocks, out, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_rows(const int32_t *pos, int32_t M, int32_t T,
+                            int32_t ntiles, int32_t *rows, hipStream_t s) {
+  const int64_t n = (int64_t)M + 1;
+  const int64_t grid = (n + kBlockThreads - 1) / kBlockThreads;
+  hipLaunchKernelGGL(k_tile_rows, dim3((unsigned)grid), dim3(kBlockThreads), 0,
+                     s, pos, M, T, ntiles, rows);
   return hipGetLastError();
 }
 

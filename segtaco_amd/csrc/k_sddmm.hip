@@ -7,8 +7,8 @@
 // a warp per row; both are disabled in the reference's build.)
 //
 // Here the entries are cut into tiles of kSdTile consecutive entries, one
-// wave per tile, and the tile's first row comes from the same K21 partition
-// pass the SpMM schedules use (k_block_starts, repaired). Each chunk of the
+// wave per tile, and the tile's first row comes from the same partition pass
+// the SpMM schedules use (k_tile_rows: the repaired K21 row). Each chunk of the
 // tile stages its column indices, values and a window of the row pointer in
 // LDS and resolves every entry's row by a binary search there. The wave is
 // P = 64/Q slots of Q lanes; a slot takes one entry, each lane multiplies V

@@ -46,6 +46,10 @@ hipError_t launch_fixup(int vec, const KArgs &a, hipStream_t s);
 hipError_t launch_block_starts(const int32_t *pos, int32_t M, int32_t T,
                                int32_t num_blocks, int32_t *out, int32_t *rows,
                                hipStream_t s);
+// rows[b] = row holding entry b*T for b < ntiles, rows[ntiles] = M: the same
+// values as launch_block_starts' `rows`, from one pass over pos.
+hipError_t launch_tile_rows(const int32_t *pos, int32_t M, int32_t T,
+                            int32_t ntiles, int32_t *rows, hipStream_t s);
 // Dynamic LDS bytes a launch of k_eb_sr / k_eb_pr uses.
 size_t eb_sr_lds_bytes(const KArgs &a);
 size_t eb_pr_lds_bytes(const KArgs &a);

@@ -3,10 +3,12 @@
 // standalone kernels (k_eb.hip) and the batched kernel (k_batch.hip).
 //
 // Stored entries are cut into tiles of T consecutive entries (tile t covers
-// [t*T, min((t+1)*T, nnz))). The partition pass (k_block_starts) computes the
-// reference's K21 array (gpu_library.h:206-215) bit-exactly and, from the same
-// search, each tile's first row after the reference's repair step that skips
-// empty rows (e.g. spmm_eb_sr.h:39-42).
+// [t*T, min((t+1)*T, nnz))). The tiles need each tile's first row after the
+// reference's repair step that skips empty rows (e.g. spmm_eb_sr.h:39-42)
+// applied to its K21 search (gpu_library.h:206-215): k_tile_rows computes
+// exactly those rows in one pass over pos; k_block_starts computes the K21
+// array itself bit-exactly (segtaco_block_starts) and the same rows from the
+// search.
 //
 // Carry-out without atomics (the reference uses atomicAdd for every row that
 // crosses a warp or block, spmm_eb_sr.h:232-256, spmm_eb_pr.h:340-371):

@@ -82,7 +82,7 @@ int32_t segtaco_sddmm(const segtaco_csr_t *S, const void *X, int64_t ldx,
   a.ntiles = (int32_t)sddmm_tiles(S);
   int32_t *tile_row = static_cast<int32_t *>(workspace);
   a.tile_row = tile_row;
-  SEGTACO_HIP(launch_block_starts(S->pos, S->M, kSdTile, a.ntiles, nullptr, tile_row, stream));
+  SEGTACO_HIP(launch_tile_rows(S->pos, S->M, kSdTile, a.ntiles, tile_row, stream));
   SEGTACO_HIP(launch_sddmm(dtype, V, a, stream));
   return SEGTACO_OK;
 }
