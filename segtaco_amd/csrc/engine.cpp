@@ -80,36 +80,37 @@ CsLayout cs_layout(int32_t nnz, int32_t N, int32_t tile, int32_t group,
 }
 
 // LDS bytes one workgroup may use on the current device (160 KiB on gfx950;
-// also the answer when no device is visible).
+// also the answer when no device is visible). Queried once per process.
 size_t device_max_lds() {
-  static size_t cached = 0;
-  if (cached > 0) return cached;
-  int dev = 0, n = 0;
-  if (hipGetDevice(&dev) == hipSuccess &&
-      hipDeviceGetAttribute(&n, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
-          hipSuccess &&
-      n > 0)
-    cached = (size_t)n;
-  else
-    cached = 160 * 1024;
-  (void)hipGetLastError();
+  static const size_t cached = [] {
+    int dev = 0, n = 0;
+    size_t v = 160 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) ==
+            hipSuccess &&
+        n > 0)
+      v = (size_t)n;
+    (void)hipGetLastError();
+    return v;
+  }();
   return cached;
 }
 
 // Compute units of the current device (256 on MI355X; also the answer when
 // no device is visible, so plans resolve the same on a host without a GPU).
+// Queried once per process.
 int32_t device_cus() {
-  static int32_t cached = 0;
-  if (cached > 0) return cached;
-  int dev = 0, n = 0;
-  if (hipGetDevice(&dev) == hipSuccess &&
-      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
-          hipSuccess &&
-      n > 0)
-    cached = n;
-  else
-    cached = 256;
-  (void)hipGetLastError();
+  static const int32_t cached = [] {
+    int dev = 0, n = 0;
+    int32_t v = 256;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) ==
+            hipSuccess &&
+        n > 0)
+      v = n;
+    (void)hipGetLastError();
+    return v;
+  }();
   return cached;
 }
 

@@ -406,15 +406,12 @@ size_t cs_lds_bytes(int32_t vec, int32_t cap) {
 
 template <int DT, int V>
 hipError_t launch_eb_cs_t(const CsArgs &a, size_t lds, hipStream_t s) {
-  // dynamic LDS above the default per-kernel limit has to be declared
-  static size_t declared = 0;
-  if (lds > declared) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eb_cs<DT, V>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)lds);
-    if (e != hipSuccess) return e;
-    declared = lds;
-  }
+  // dynamic LDS above the default per-kernel limit has to be declared; the
+  // capacity is fixed per device, so once per instantiation is enough
+  static const hipError_t declared = hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&k_eb_cs<DT, V>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (declared != hipSuccess) return declared;
   const unsigned grid = (unsigned)((a.units + a.rounds - 1) / a.rounds);
   hipLaunchKernelGGL((k_eb_cs<DT, V>), dim3(grid), dim3(kCsThreads), lds, s, a);
   return hipGetLastError();
