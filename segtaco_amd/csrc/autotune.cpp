@@ -244,10 +244,11 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
       make(SEGTACO_RB_SR, 0, 0, 0, 0),
       make(SEGTACO_RB_SR, 0, 0, 4, 0),
       make(SEGTACO_RB_PR, 0, 0, 0, 0),
-      make(SEGTACO_RB_PR, 0, 0, 4, 0),
+      make(SEGTACO_RB_PR, 0, 0, 8, 0),
       make(SEGTACO_EB_SR, 0, 0, 0, 64),
       make(SEGTACO_EB_SR, 0, 0, 0, 128),
       make(SEGTACO_EB_SR, 0, 0, 0, 256),
+      make(SEGTACO_EB_PR, 0, 0, 0, 0),
       make(SEGTACO_EB_PR, 0, 0, 0, 256),
   };
   std::vector<segtaco_plan_t> plans;
