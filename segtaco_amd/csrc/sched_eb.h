@@ -394,7 +394,21 @@ __device__ __forceinline__ void fixup_body(const KArgs &a, int32_t blk,
   const int32_t t0 = rs / a.tile;
   float acc[V];
   load_f32vec<V>(a.carry_tail + (int64_t)t0 * a.N + col, acc);
-  for (int32_t tt = t0 + 1; tt <= t; ++tt) {
+  // A long row spans many tiles: load its heads eight at a time (all in
+  // flight together), then add them one by one in tile order.
+  constexpr int kH = 8;
+  int32_t tt = t0 + 1;
+  for (; tt + kH - 1 <= t; tt += kH) {
+    float h[kH][V];
+#pragma unroll
+    for (int k = 0; k < kH; ++k)
+      load_f32vec<V>(a.carry_head + (int64_t)(tt + k) * a.N + col, h[k]);
+#pragma unroll
+    for (int k = 0; k < kH; ++k)
+#pragma unroll
+      for (int j = 0; j < V; ++j) acc[j] += h[k][j];
+  }
+  for (; tt <= t; ++tt) {
     float h[V];
     load_f32vec<V>(a.carry_head + (int64_t)tt * a.N + col, h);
 #pragma unroll
