@@ -77,7 +77,7 @@ const char *segtaco_legacy_name(int32_t id);
 void segtaco_legacy_call(int32_t id, taco_tensor_t *C, taco_tensor_t *A,
                          taco_tensor_t *B);
 
-/* The five north-star entry points. */
+/* The five primary entry points (the four schedules and the TACO kernel). */
 /* spmm_rb_sr.h:220-243: row-balanced, sequential reduction. */
 void spmm_rb_sr(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B);
 /* spmm_rb_pr.h:333-375: row-balanced, parallel (tree) reduction. */

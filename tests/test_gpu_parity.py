@@ -228,7 +228,7 @@ def test_stage_times_run_the_same_product(sched):
 
 @pytest.mark.parametrize("name", [
     "spmm_rb_sr", "spmm_rb_pr", "spmm_eb_sr", "spmm_eb_pr", "spmm_csr_gpu_taco"])
-def test_north_star_entry_points(name):
+def test_primary_entry_points(name):
     pos, crd, vals, B = _case("empty_runs_n20")
     C, ms = S.legacy(name, pos, crd, vals, B)
     _check(pos, crd, vals, B, C, what=name)
