@@ -246,7 +246,7 @@ def best_form(res):
             "hbm_frac": round(res[b]["hbm_frac"], 4)}
 
 
-def cfg4_set(n, seed=41):
+def powerlaw_batch_set(n, seed=41):
     """The batching workload: n CSR matrices, M ~ U[1000, 200000] (square),
     log-normal row lengths with mean ~ U[8, 64] (sigma 1.5, cap 8192), each
     with its own N drawn from {16, 32, 64, 128}."""
@@ -550,7 +550,7 @@ def main():
         out["forms"] = forms
 
     if not args.no_extra and args.batch_matrices > 0:
-        mats = cfg4_set(args.batch_matrices)
+        mats = powerlaw_batch_set(args.batch_matrices)
         out["batched"] = {"f32": batch_leg(torch, mats, "f32", 5, 2, stream),
                           "bf16": batch_leg(torch, mats, "bf16", 5, 2, stream)}
         out["served"] = serve_leg(torch, mats, "bf16", args.serve_seconds, 2000.0, pl)

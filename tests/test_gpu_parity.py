@@ -68,7 +68,7 @@ def _case(name):
         d = np.load(name.split(":", 1)[1], allow_pickle=False)
         vals = d["vals"] if int(d["has_vals"]) else None
         return d["pos"], d["crd"], vals, d["B"]
-    if name == "cfg1_uniform_4096_n32":
+    if name == "uniform_4096_n32":
         pos, crd, vals = S.gen_csr(S.GEN_UNIFORM, 1, 4096, 4096, mean=16)
         return pos, crd, vals, S.gen_dense(11, 4096, 32)
     if name == "powerlaw_20000_n128":
@@ -116,7 +116,7 @@ def _mk(rng, lens, K, N):
 
 
 CASES = ([f"golden:{p}" for p in GOLDEN] +
-         ["cfg1_uniform_4096_n32", "powerlaw_20000_n128", "empty_runs_n20", "one_long_row_n130",
+         ["uniform_4096_n32", "powerlaw_20000_n128", "empty_runs_n20", "one_long_row_n130",
           "spmv_n1", "n3_odd", "n512_panels", "single_row", "all_empty_n8", "novals_n16"])
 CASE_IDS = [c.split("/")[-1] for c in CASES]
 
@@ -237,7 +237,7 @@ def test_primary_entry_points(name):
 
 def test_all_operator_surface_names():
     pos, crd, vals, B = _case("n3_odd")
-    pos2, crd2, vals2, B2 = _case("cfg1_uniform_4096_n32")
+    pos2, crd2, vals2, B2 = _case("uniform_4096_n32")
     for name in S.legacy_names():
         print(f"[legacy] {name}", flush=True)
         C, _ = S.legacy(name, pos, crd, vals, B)
