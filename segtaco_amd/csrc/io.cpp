@@ -196,8 +196,14 @@ int32_t segtaco_mtx_read(const char *path, int32_t flags, int32_t *nrow,
       return set_error(SEGTACO_ERR_INVALID, "mtx: entry %lld (%lld, %lld) outside %lld x %lld",
                        i + 1, (long long)r, (long long)c, m, n);
     keys.push_back(key_of((int32_t)(r - 1), (int32_t)(c - 1)));
-    if (info.symmetry == 1 && r != c)
+    if (info.symmetry == 1 && r != c) {
+      // the mirrored entry must fit too (a non-square "symmetric" file)
+      if (c > m || r > n)
+        return set_error(SEGTACO_ERR_INVALID,
+                         "mtx: entry %lld (%lld, %lld) mirrors outside %lld x %lld", i + 1,
+                         (long long)r, (long long)c, m, n);
       keys.push_back(key_of((int32_t)(c - 1), (int32_t)(r - 1)));
+    }
   }
   std::sort(keys.begin(), keys.end());
   if (info.symmetry == 1) {

@@ -116,6 +116,8 @@ def test_symmetric_expansion_is_complete_by_default(tmp_path):
     ("%%MatrixMarket matrix coordinate real general\n3 3 1\n0 1 1\n", "range"),
     ("%%MatrixMarket matrix coordinate real general\n3 3 1\n1 1\n", "value"),
     ("%%MatrixMarket matrix coordinate real general\n% only comments\n", "size"),
+    # a non-square "symmetric" file whose mirrored entry (2, 5) leaves 5 x 3
+    ("%%MatrixMarket matrix coordinate real symmetric\n5 3 1\n5 2 1\n", "mirror"),
 ])
 def test_malformed_files_are_errors(tmp_path, text, what):
     path = tmp_path / "bad.mtx"
