@@ -353,7 +353,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     char *w = static_cast<char *>(ws);
     a.ntiles = L.eb.ntiles;
     a.rows_per_slice = (int32_t)ceil_div(A->M, L.eb.ntiles);
-    a.part = reinterpret_cast<int32_t *>(w + L.eb.part_off);
+    a.fix_t0 = reinterpret_cast<int32_t *>(w + L.eb.part_off);
     a.carry_head = reinterpret_cast<float *>(w + L.eb.head_off);
     a.carry_tail = reinterpret_cast<float *>(w + L.eb.tail_off);
     a.tile_row = reinterpret_cast<int32_t *>(w + L.eb.rows_off);
@@ -391,8 +391,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     a.lq = ilog2(natural_lanes(N, V));
     a.panels = (int32_t)ceil_div(N, (int64_t)(1 << a.lq) * V);
     SEGTACO_HIP(mark(0));
-    SEGTACO_HIP(launch_tile_rows(A->pos, A->M, plan.tile, L.eb.ntiles,
-                                 const_cast<int32_t *>(a.tile_row), stream));
+    SEGTACO_HIP(launch_tile_rows_fix(V, a, stream));
     SEGTACO_HIP(launch_cs_prep(c, stream));
     SEGTACO_HIP(mark(1));
     SEGTACO_HIP(launch_eb_cs(b_dtype, V, c, stream));
@@ -410,13 +409,12 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   char *w = static_cast<char *>(ws);
   a.ntiles = L.ntiles;
   a.rows_per_slice = (int32_t)ceil_div(A->M, L.ntiles);
-  a.part = reinterpret_cast<int32_t *>(w + L.part_off);
+  a.fix_t0 = reinterpret_cast<int32_t *>(w + L.part_off);
   a.carry_head = reinterpret_cast<float *>(w + L.head_off);
   a.carry_tail = reinterpret_cast<float *>(w + L.tail_off);
   a.tile_row = reinterpret_cast<int32_t *>(w + L.rows_off);
   SEGTACO_HIP(mark(0));
-  SEGTACO_HIP(launch_tile_rows(A->pos, A->M, plan.tile, L.ntiles,
-                               const_cast<int32_t *>(a.tile_row), stream));
+  SEGTACO_HIP(launch_tile_rows_fix(V, a, stream));
   SEGTACO_HIP(mark(1));
   if (plan.schedule == SEGTACO_EB_SR) {
     const int64_t tiles_per_block =

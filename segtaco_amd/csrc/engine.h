@@ -43,7 +43,9 @@ inline int64_t pow2_ceil(int64_t x) {
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 inline int32_t elem_bytes(int32_t dt) { return dt == SEGTACO_DT_F32 ? 4 : 2; }
 
-// Workspace layout of an element-balanced plan.
+// Workspace layout of an element-balanced plan: ntiles + 1 ints at part_off
+// (a batch keeps the K21 partition there, a standalone call the fix-up's
+// per-tile records), ntiles + 1 tile rows, then the two carry arrays.
 struct EbLayout {
   int32_t ntiles;
   size_t part_off, rows_off, head_off, tail_off, total;

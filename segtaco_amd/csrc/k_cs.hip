@@ -23,8 +23,8 @@
 //     unit's LDS accumulators, let every lane group sweep its sorted tile —
 //     interior rows accumulate in LDS, head and tail rows in registers —
 //     then write the unit's interior rows to C and the head/tail partials to
-//     the carry arrays; k_fixup (sched_eb.h) combines carries and zero-fills
-//     empty rows exactly as for eb_sr.
+//     the carry arrays; as for eb_sr, k_tile_rows_fix (k_eb.hip) zeroes the
+//     empty rows and k_fixup (sched_eb.h) combines the carries.
 // Each row's entries still meet its accumulator in ascending column order,
 // which is CSR order for canonical CSR, with one fmaf per entry starting from
 // zero: C is bit-identical to eb_sr with the same tile size. A unit whose

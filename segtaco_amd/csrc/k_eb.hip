@@ -65,6 +65,47 @@ __global__ __launch_bounds__(kBlockThreads) void k_tile_rows(
   }
 }
 
+// k_tile_rows plus the fix-up's bookkeeping, for a standalone call:
+// fix_t0[b] = the tile where the row ending in tile b started, if it started
+// before tile b (the fix-up then sums that row's carries), else -1; and every
+// empty row of C is written as zeros here, since no later kernel of the call
+// writes it. Each of tiles ceil(p0/T)..floor((p1-1)/T) starts inside row r,
+// so every tile's entries are written exactly once. One thread per row; a
+// wave zeroes its empty rows together, 64 lanes x V floats across a row.
+template <int V>
+__global__ __launch_bounds__(kBlockThreads) void k_tile_rows_fix(
+    const int32_t *__restrict__ pos, int32_t M, int32_t T, int32_t ntiles,
+    int32_t *__restrict__ rows, int32_t *__restrict__ fix_t0,
+    float *__restrict__ C, int32_t N, int64_t ldc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool empty = false;
+  if (r < M) {
+    const int32_t p0 = pos[r], p1 = pos[r + 1];
+    if (p0 < p1) {
+      const int32_t b1 = (p1 - 1) / T;
+      for (int32_t b = (int32_t)(((int64_t)p0 + T - 1) / T); b <= b1; ++b) {
+        rows[b] = (int32_t)r;
+        fix_t0[b] = (b == b1 && (int64_t)p0 < (int64_t)b * T) ? p0 / T : -1;
+      }
+    } else {
+      empty = true;
+    }
+  } else if (r == M) {
+    rows[ntiles] = M;
+  }
+  unsigned long long m = __ballot(empty);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave_row0 = r - lane;
+  float z[V];
+  zero_vec(z);
+  while (m) {
+    const int i = __ffsll(m) - 1;
+    m &= m - 1;
+    float *row = C + (wave_row0 + i) * ldc;
+    for (int32_t j = lane * V; j < N; j += 64 * V) store_vec<V>(row + j, z);
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_fill_zero(float *C,
                                                              int32_t M,
                                                              int32_t N,
@@ -149,6 +190,21 @@ hipError_t launch_tile_rows(const int32_t *pos, int32_t M, int32_t T,
   const int64_t grid = (n + kBlockThreads - 1) / kBlockThreads;
   hipLaunchKernelGGL(k_tile_rows, dim3((unsigned)grid), dim3(kBlockThreads), 0,
                      s, pos, M, T, ntiles, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_rows_fix(int vec, const KArgs &a, hipStream_t s) {
+  const int64_t n = (int64_t)a.M + 1;
+  const dim3 g((unsigned)((n + kBlockThreads - 1) / kBlockThreads)), b(kBlockThreads);
+  int32_t *rows = const_cast<int32_t *>(a.tile_row);
+  int32_t *fix = const_cast<int32_t *>(a.fix_t0);
+  switch (vec) {
+    case 1: hipLaunchKernelGGL((k_tile_rows_fix<1>), g, b, 0, s, a.pos, a.M, a.tile, a.ntiles, rows, fix, a.C, a.N, a.ldc); break;
+    case 2: hipLaunchKernelGGL((k_tile_rows_fix<2>), g, b, 0, s, a.pos, a.M, a.tile, a.ntiles, rows, fix, a.C, a.N, a.ldc); break;
+    case 4: hipLaunchKernelGGL((k_tile_rows_fix<4>), g, b, 0, s, a.pos, a.M, a.tile, a.ntiles, rows, fix, a.C, a.N, a.ldc); break;
+    case 8: hipLaunchKernelGGL((k_tile_rows_fix<8>), g, b, 0, s, a.pos, a.M, a.tile, a.ntiles, rows, fix, a.C, a.N, a.ldc); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

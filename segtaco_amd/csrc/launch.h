@@ -31,6 +31,10 @@ struct KArgs {
   int32_t rows_per_slice;  // eb_*: rows per tile for empty-row zero fill
   const int32_t *part;     // eb_*: K21 partition, ntiles + 1 entries
   const int32_t *tile_row; // eb_*: row holding each tile's first entry
+  // eb_*: per tile, the tile where the row that ends in it started, if that
+  // row began in an earlier tile, else -1 (written by k_tile_rows_fix, which
+  // also zeroes the empty rows of C); null: the fix-up derives both from pos
+  const int32_t *fix_t0;
   float *carry_head;       // eb_*: [ntiles][N] partial of a continued row
   float *carry_tail;       // eb_*: [ntiles][N] partial of a row left open
 };
@@ -50,6 +54,10 @@ hipError_t launch_block_starts(const int32_t *pos, int32_t M, int32_t T,
 // values as launch_block_starts' `rows`, from one pass over pos.
 hipError_t launch_tile_rows(const int32_t *pos, int32_t M, int32_t T,
                             int32_t ntiles, int32_t *rows, hipStream_t s);
+// launch_tile_rows for a standalone SpMM call: also writes a.fix_t0 and zeroes
+// the empty rows of a.C (vec floats per store), so that the fix-up only sums
+// the carries of rows that span tiles.
+hipError_t launch_tile_rows_fix(int vec, const KArgs &a, hipStream_t s);
 // Dynamic LDS bytes a launch of k_eb_sr / k_eb_pr uses.
 size_t eb_sr_lds_bytes(const KArgs &a);
 size_t eb_pr_lds_bytes(const KArgs &a);

@@ -17,9 +17,10 @@
 //     carry_head[t]; a row that starts in tile t and continues past it
 //     contributes to carry_tail[t];
 //   * k_fixup writes every row that spans tiles t0..t1 as
-//     carry_tail[t0] + carry_head[t0+1] + ... + carry_head[t1], in tile order,
-//     and writes the empty rows of tile t's row slice
-//     [t*rows_per_slice, (t+1)*rows_per_slice) as zeros.
+//     carry_tail[t0] + carry_head[t0+1] + ... + carry_head[t1], in tile order;
+//   * the empty rows are written as zeros: by k_tile_rows_fix before the main
+//     kernel in a standalone call, by the fix-up (tile t zeroes the empty rows
+//     of its slice [t*rows_per_slice, (t+1)*rows_per_slice)) in a batch.
 // Every C element has exactly one writer and a fixed summation order: results
 // are bit-identical run to run, and C needs no clearing.
 //
@@ -151,7 +152,7 @@ __device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
   auto next_row = [&](int32_t pp) {
     flush();
     ++r;
-    while (P(r + 1) <= pp) ++r;  // skip empty rows (zeroed by k_fixup)
+    while (P(r + 1) <= pp) ++r;  // skip empty rows (zeroed elsewhere)
     rs = P(r);
     re = P(r + 1);
     comp = (re - rs) > a.comp_threshold;
@@ -365,37 +366,15 @@ __device__ __forceinline__ void eb_pr_body(const KArgs &a, int32_t blk,
   }
 }
 
-// Per tile t (one lane group): zero the empty rows of its row slice, then, if
-// the tile's first row started in an earlier tile t0 and ends in t, write
-// C[r] = carry_tail[t0] + carry_head[t0+1] + ... + carry_head[t].
+// C[r][col..col+V) = carry_tail[t0] + carry_head[t0+1] + ... + carry_head[t],
+// added in tile order. A long row spans many tiles: its heads are loaded
+// eight at a time (all in flight together), then added one by one.
 template <int V>
-__device__ __forceinline__ void fixup_body(const KArgs &a, int32_t blk,
-                                           int32_t panel) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int lq = a.lq;
-  const int32_t Q = 1 << lq;
-  const int32_t q = lane & (Q - 1);
-  const int32_t t = ((blk * kWavesPerBlock + wave) << (6 - lq)) + (lane >> lq);
-  if (t >= a.ntiles) return;
-  {
-    const int64_t r0 = (int64_t)t * a.rows_per_slice;
-    const int64_t r1 = r0 + a.rows_per_slice < a.M ? r0 + a.rows_per_slice : a.M;
-    if (r0 < r1) zero_empty_rows<V>(a, (int32_t)r0, (int32_t)r1, q, Q, panel);
-  }
-  const int32_t col = ((panel << lq) + q) * V;
-  if (t == 0 || col >= a.N) return;
-  const int32_t *__restrict__ pos = a.pos;
-  const int32_t s = t * a.tile;
-  const int32_t e = (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile);
-  const int32_t r = a.tile_row[t];
-  const int32_t rs = pos[r], re = pos[r + 1];
-  if (!(rs < s && re <= e)) return;
-  const int32_t t0 = rs / a.tile;
+__device__ __forceinline__ void combine_carries(const KArgs &a, int32_t t0,
+                                                int32_t t, int32_t r,
+                                                int32_t col) {
   float acc[V];
   load_f32vec<V>(a.carry_tail + (int64_t)t0 * a.N + col, acc);
-  // A long row spans many tiles: load its heads eight at a time (all in
-  // flight together), then add them one by one in tile order.
   constexpr int kH = 8;
   int32_t tt = t0 + 1;
   for (; tt + kH - 1 <= t; tt += kH) {
@@ -415,6 +394,43 @@ __device__ __forceinline__ void fixup_body(const KArgs &a, int32_t blk,
     for (int j = 0; j < V; ++j) acc[j] += h[j];
   }
   store_vec<V>(a.C + (int64_t)r * a.ldc + col, acc);
+}
+
+// Per tile t (one lane group). With a.fix_t0 (standalone calls, where
+// k_tile_rows_fix already zeroed the empty rows): if a row that started in
+// tile t0 < t ends in t, write it from its carries. Without (batched calls,
+// whose partition is made once and reused): zero the empty rows of t's row
+// slice, then find such a row from tile_row and pos and write it the same way.
+template <int V>
+__device__ __forceinline__ void fixup_body(const KArgs &a, int32_t blk,
+                                           int32_t panel) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lq = a.lq;
+  const int32_t Q = 1 << lq;
+  const int32_t q = lane & (Q - 1);
+  const int32_t t = ((blk * kWavesPerBlock + wave) << (6 - lq)) + (lane >> lq);
+  if (t >= a.ntiles) return;
+  const int32_t col = ((panel << lq) + q) * V;
+  if (a.fix_t0) {
+    const int32_t t0 = a.fix_t0[t];
+    const int32_t r = a.tile_row[t];
+    if (t0 >= 0 && col < a.N) combine_carries<V>(a, t0, t, r, col);
+    return;
+  }
+  {
+    const int64_t r0 = (int64_t)t * a.rows_per_slice;
+    const int64_t r1 = r0 + a.rows_per_slice < a.M ? r0 + a.rows_per_slice : a.M;
+    if (r0 < r1) zero_empty_rows<V>(a, (int32_t)r0, (int32_t)r1, q, Q, panel);
+  }
+  if (t == 0 || col >= a.N) return;
+  const int32_t *__restrict__ pos = a.pos;
+  const int32_t s = t * a.tile;
+  const int32_t e = (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile);
+  const int32_t r = a.tile_row[t];
+  const int32_t rs = pos[r], re = pos[r + 1];
+  if (!(rs < s && re <= e)) return;
+  combine_carries<V>(a, rs / a.tile, t, r, col);
 }
 
 }  // namespace segtaco
