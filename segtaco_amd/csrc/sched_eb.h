@@ -96,8 +96,13 @@ __device__ __forceinline__ void zero_empty_rows(const KArgs &a, int32_t r0,
 
 // eb_sr over tile block `blk` (kWavesPerBlock * 64/Q tiles, one per lane
 // group), column panel `panel`; smem holds eb_sr_lds_bytes_lq(lq) bytes.
-template <int DT, int V>
-__device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
+// kWide: some tile of the wave spans more rows than the chunk's LDS window of
+// the row pointer holds, so row lookups past the window read pos from memory.
+// Otherwise every lookup is an LDS read: a lookup that may go to either
+// address space compiles to a flat load, whose wait drains every gather the
+// lane group has in flight.
+template <int DT, int V, bool kWide>
+__device__ __forceinline__ void eb_sr_walk(const KArgs &a, int32_t blk,
                                            int32_t panel, int32_t *smem) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -132,9 +137,11 @@ __device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
   zero_vec(cmp);
   int32_t rbase = 0, pw = 0;
 
+  // pos[row]: from the chunk's LDS window (rows rbase .. rbase+pw-1)
   auto P = [&](int32_t row) -> int32_t {
     const int32_t i = row - rbase;
-    return i < pw ? s_pos[i] : pos[row];
+    if constexpr (kWide) return i < pw ? s_pos[i] : pos[row];
+    return s_pos[i];
   };
   auto flush = [&]() {
     if (!colok) return;
@@ -182,15 +189,19 @@ __device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
         comp = (re - rs) > a.comp_threshold;
       }
       // Double-buffered: batch k+1's gathers are in flight while batch k is
-      // consumed, so the memory pipe never drains between batches.
+      // consumed, so the memory pipe never drains between batches. Every
+      // batch issues exactly kSrBatch loads, with no branch around them
+      // (entries past the chunk and columns past N re-load the chunk's last
+      // entry / column 0, and are not consumed): the number of loads in flight
+      // is then the same on every path, so the wait before consuming a batch
+      // only covers that batch and leaves the next one in flight.
       R ra[kSrBatch], rb[kSrBatch];
+      const int32_t col_ld = colok ? col : 0;
       auto issue = [&](R (&buf)[kSrBatch], int32_t b0) {
-        const int32_t nb = min(kSrBatch, cn - b0);
-        if (colok) {
 #pragma unroll
-          for (int u = 0; u < kSrBatch; ++u)
-            if (u < nb)
-              buf[u] = load_raw<DT, V>(B, (int64_t)s_crd[b0 + u] * ldb + col);
+        for (int u = 0; u < kSrBatch; ++u) {
+          const int32_t i = min(b0 + u, cn - 1);
+          buf[u] = load_raw<DT, V>(B, (int64_t)s_crd[i] * ldb + col_ld);
         }
       };
       auto consume = [&](const R (&buf)[kSrBatch], int32_t b0) {
@@ -215,17 +226,41 @@ __device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
           }
         }
       };
+      // the final pair of batches issues nothing after it
+      const int32_t last = (cn - 1) / (2 * kSrBatch) * (2 * kSrBatch);
       issue(ra, 0);
-      for (int32_t b0 = 0; b0 < cn; b0 += 2 * kSrBatch) {
-        if (b0 + kSrBatch < cn) issue(rb, b0 + kSrBatch);
+      for (int32_t b0 = 0; b0 < last; b0 += 2 * kSrBatch) {
+        issue(rb, b0 + kSrBatch);
         consume(ra, b0);
-        if (b0 + 2 * kSrBatch < cn) issue(ra, b0 + 2 * kSrBatch);
-        if (b0 + kSrBatch < cn) consume(rb, b0 + kSrBatch);
+        issue(ra, b0 + 2 * kSrBatch);
+        consume(rb, b0 + kSrBatch);
       }
+      issue(rb, last + kSrBatch);
+      consume(ra, last);
+      consume(rb, last + kSrBatch);
     }
     wave_sync();
   }
   if (tok) flush();
+}
+
+// A lane group's lookups stay inside its LDS window when its tile's rows
+// r0 = tile_row[t] .. r1 = tile_row[t+1] satisfy r1 - r0 <= PW - 2: each chunk's
+// window starts at a row >= r0, and the walk reads pos at rows up to
+// min(r1, M-1) + 1 (the row after the one holding the tile's last entry).
+template <int DT, int V>
+__device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
+                                           int32_t panel, int32_t *smem) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lq = a.lq;
+  const int32_t t = blk * (kWavesPerBlock << (6 - lq)) + (wave << (6 - lq)) + (lane >> lq);
+  const int32_t PW = eb_sr_chunk(lq) + kPosSlack;
+  const bool wide = t < a.ntiles && a.tile_row[t + 1] - a.tile_row[t] > PW - 2;
+  if (__ballot(wide))
+    eb_sr_walk<DT, V, true>(a, blk, panel, smem);
+  else
+    eb_sr_walk<DT, V, false>(a, blk, panel, smem);
 }
 
 // eb_pr over tile block `blk` (one tile per wave), column panel `panel`;

@@ -23,6 +23,9 @@ struct SddmmArgs {
   int32_t tile;      // entries per tile (kSdTile)
   int32_t ntiles;
   const int32_t *tile_row;  // row holding each tile's first entry (ntiles+1)
+  // row * ld + column (elements) of X and Y fits 32 bits after scaling to
+  // bytes, and every row index and leading dimension is below 2^24
+  int32_t off32;
 };
 
 int32_t sddmm_tile();

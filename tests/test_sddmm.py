@@ -142,12 +142,15 @@ def _case(name):
         return (*_pattern(rng, rng.integers(0, 50, 1000), 900), 900, 1)
     if name == "j300_multi_chunk":
         return (*_pattern(rng, rng.integers(0, 16, 800), 700), 700, 300)
+    if name == "j8_narrow":  # 2 lanes x 4 (FP32) or 1 lane x 8 per entry
+        return (*_pattern(rng, rng.integers(0, 40, 900), 800), 800, 8)
     if name == "all_empty":
         return (*_pattern(rng, np.zeros(64, int), 10), 10, 8)
     raise KeyError(name)
 
 
-CASES = ["ragged_j128", "long_rows_j64", "j37_odd", "j1", "j300_multi_chunk", "all_empty"]
+CASES = ["ragged_j128", "long_rows_j64", "j37_odd", "j1", "j8_narrow", "j300_multi_chunk",
+         "all_empty"]
 
 
 @pytest.mark.gpu
