@@ -447,6 +447,9 @@ def main():
                          "each 'schedule[:key=value[,key=value]]', e.g. 'eb_sr:tile=256'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline leg only")
+    ap.add_argument("--extra", default="vendor,sddmm,forms,batched",
+                    help="comma-separated legs after the headline: vendor (rocSPARSE), sddmm, "
+                         "forms (N=256 storage types, banded), batched (with serving)")
     ap.add_argument("--batch-matrices", type=int, default=512,
                     help="products in the batching / serving workload (0 skips both)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
@@ -523,13 +526,16 @@ def main():
                           "hbm_frac": round(v["hbm_frac"], 4)} for k, v in head.items()},
     }
 
-    if not args.no_extra:
+    legs = set() if args.no_extra else {x for x in args.extra.split(",") if x}
+    if "vendor" in legs:
         vend = vendor_leg(torch, pl, N, args.steps, stream)
         ok = [v["gflops"] for v in vend.values() if "gflops" in v]
         out["vendor_rocsparse"] = vend
         out["vs_rocsparse_best"] = round(out["value"] / max(ok), 3) if ok else None
+    if "sddmm" in legs:
         out["sddmm"] = {d: sddmm_leg(torch, pl, 128, d, args.steps, args.warmup, stream)
                         for d in ("f32", "bf16")}
+    if "forms" in legs:
         forms = {}
         for dtype in ("f32", "bf16", "fp16"):
             res, _ = run_leg(torch, pl, 256, dtype, [("eb_sr", S.make_plan("eb_sr")),
@@ -549,13 +555,14 @@ def main():
         forms["bf16_vs_f32_n256"] = round(forms["powerlaw_n256_bf16"]["gflops"] / f32, 3)
         out["forms"] = forms
 
-    if not args.no_extra and args.batch_matrices > 0:
+    if "batched" in legs and args.batch_matrices > 0:
         mats = powerlaw_batch_set(args.batch_matrices)
         out["batched"] = {"f32": batch_leg(torch, mats, "f32", 5, 2, stream),
                           "bf16": batch_leg(torch, mats, "bf16", 5, 2, stream)}
         out["served"] = serve_leg(torch, mats, "bf16", args.serve_seconds, 2000.0, pl)
         sat = out["served"]["powerlaw_n128_saturated"]["gflops"]
-        out["forms"]["served_bf16_powerlaw_vs_plain_f32"] = round(sat / out["value"], 3)
+        out.setdefault("forms", {})["served_bf16_powerlaw_vs_plain_f32"] = round(
+            sat / out["value"], 3)
         del mats
 
     if not args.no_cpu_baseline:

@@ -237,10 +237,10 @@ int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                      const segtaco_plan_t *plan, void *workspace,
                      size_t workspace_bytes, void *stream);
 
-/* Runs segtaco_spmm `reps` times (synchronously) and returns the mean ms of
- * its three stages in out_ms[0..2]: row partition, main kernel, carry fix-up
- * (0 for stages a plan does not have), each timed with HIP events on
- * `stream`. */
+/* Runs segtaco_spmm `reps` times back to back on `stream` and returns the
+ * mean ms of its three stages in out_ms[0..2]: row partition, main kernel,
+ * carry fix-up (0 for stages a plan does not have), each timed with HIP
+ * events on `stream`. Returns after the last run completed. */
 int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,
                                  int32_t b_dtype, int64_t ldb, int32_t N,
                                  float *C, int64_t ldc,

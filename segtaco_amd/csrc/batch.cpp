@@ -93,6 +93,8 @@ void fill_batch_desc(const segtaco_csr_t &A, const void *B, int64_t ldb,
   a.vals = A.vals;
   a.B = B;
   a.ldb = ldb;
+  // judged for 4-byte elements, which covers the 16-bit storage types too
+  a.b_off32 = b_off32_ok(A.K, ldb, 4);
   a.C = C;
   a.ldc = ldc;
   a.lq = ilog2(plan.col_lanes);

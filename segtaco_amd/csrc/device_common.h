@@ -150,6 +150,31 @@ __device__ __forceinline__ void unpack(const typename RawT<DT, V>::type &x,
   }
 }
 
+// V elements of row `row` of a row-major matrix (leading dimension ld,
+// elements) from column c. kOff32: row, ld < 2^24 and every byte offset of
+// the matrix fits 32 bits, so the offset is one 24-bit multiply added to the
+// uniform base as a 32-bit byte offset (global_load with a scalar base)
+// instead of 64-bit integer arithmetic with quarter-rate multiplies.
+template <int DT, int V, bool kOff32>
+__device__ __forceinline__ typename RawT<DT, V>::type load_row_raw(
+    const void *__restrict__ base, int32_t row, int64_t ld, int32_t c) {
+  if constexpr (kOff32) {
+    constexpr uint32_t kEb = DT == 0 ? 4 : 2;
+    const uint32_t off = (__umul24((uint32_t)row, (uint32_t)ld) + (uint32_t)c) * kEb;
+    return *reinterpret_cast<const typename RawT<DT, V>::type *>(
+        static_cast<const char *>(base) + off);
+  } else {
+    return load_raw<DT, V>(base, (int64_t)row * ld + c);
+  }
+}
+
+template <int DT, int V, bool kOff32>
+__device__ __forceinline__ void load_row_vec(const void *__restrict__ base,
+                                             int32_t row, int64_t ld, int32_t c,
+                                             float (&out)[V]) {
+  unpack<DT, V>(load_row_raw<DT, V, kOff32>(base, row, ld, c), out);
+}
+
 // Makes this wave's LDS writes visible to all of its lanes: a release/acquire
 // fence at wavefront scope around a wave barrier (no block-wide s_barrier).
 __device__ __forceinline__ void wave_sync() {

@@ -5,6 +5,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "cs.h"
 #include "launch.h"
@@ -306,6 +307,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   a.vals = A->vals;
   a.B = B;
   a.ldb = ldb;
+  a.b_off32 = b_off32_ok(A->K, ldb, elem_bytes(b_dtype));
   a.C = C;
   a.ldc = ldc;
   a.lq = ilog2(plan.col_lanes);
@@ -492,25 +494,33 @@ int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,
   if (!out_ms || reps < 1)
     return set_error(SEGTACO_ERR_INVALID, "segtaco_spmm_stage_times: bad argument");
   hipStream_t stream = static_cast<hipStream_t>(stream_v);
-  hipEvent_t ev[4] = {};
-  for (auto &e : ev) SEGTACO_HIP(hipEventCreate(&e));
-  double acc[3] = {0, 0, 0};
+  // All reps are enqueued back to back (as in a steady loop of calls), each
+  // with its own four events; the host waits once at the end.
+  std::vector<hipEvent_t> ev((size_t)reps * 4, nullptr);
   int32_t st = SEGTACO_OK;
-  for (int32_t r = 0; r < reps && st == SEGTACO_OK; ++r) {
-    st = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace,
-                   workspace_bytes, stream, ev);
-    if (st != SEGTACO_OK) break;
-    if (hipEventSynchronize(ev[3]) != hipSuccess) {
-      st = set_error(SEGTACO_ERR_HIP, "hipEventSynchronize");
+  for (auto &e : ev) {
+    const hipError_t he = hipEventCreate(&e);
+    if (he != hipSuccess) {
+      e = nullptr;
+      st = hip_error(he, "hipEventCreate");
       break;
     }
+  }
+  for (int32_t r = 0; r < reps && st == SEGTACO_OK; ++r)
+    st = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace,
+                   workspace_bytes, stream, ev.data() + 4 * (size_t)r);
+  if (st == SEGTACO_OK && hipEventSynchronize(ev.back()) != hipSuccess)
+    st = set_error(SEGTACO_ERR_HIP, "hipEventSynchronize");
+  double acc[3] = {0, 0, 0};
+  for (int32_t r = 0; r < reps && st == SEGTACO_OK; ++r)
     for (int i = 0; i < 3; ++i) {
       float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      (void)hipEventElapsedTime(&ms, ev[4 * (size_t)r + i], ev[4 * (size_t)r + i + 1]);
       acc[i] += ms;
     }
-  }
-  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (st != SEGTACO_OK) (void)hipStreamSynchronize(stream);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (st != SEGTACO_OK) return st;
   for (int i = 0; i < 3; ++i) out_ms[i] = (float)(acc[i] / reps);
   return SEGTACO_OK;

@@ -30,23 +30,6 @@ constexpr int32_t kSdChunk = 128;
 constexpr int32_t kSdPosSlack = 8;
 constexpr int kSdU = 4;  // consecutive entries per slot and batch
 
-// V elements of row `row` of a row-major matrix at column c. kOff32: the
-// element offset row * ld + c fits 32 bits and row, ld < 2^24, so it is
-// formed with a 24-bit multiply and added to the uniform base as a 32-bit
-// byte offset; otherwise in 64 bits.
-template <int DT, int V, bool kOff32>
-__device__ __forceinline__ typename RawT<DT, V>::type load_row_raw(
-    const void *__restrict__ base, int32_t row, int64_t ld, int32_t c) {
-  if constexpr (kOff32) {
-    constexpr uint32_t kEb = DT == 0 ? 4 : 2;
-    const uint32_t off = (__umul24((uint32_t)row, (uint32_t)ld) + (uint32_t)c) * kEb;
-    return *reinterpret_cast<const typename RawT<DT, V>::type *>(
-        static_cast<const char *>(base) + off);
-  } else {
-    return load_raw<DT, V>(base, (int64_t)row * ld + c);
-  }
-}
-
 template <int DT, int V, bool kOff32>
 __global__ __launch_bounds__(256) void k_sddmm(const SddmmArgs a) {
   __shared__ int32_t smem[4 * (4 * kSdChunk + kSdPosSlack)];

@@ -37,7 +37,17 @@ struct KArgs {
   const int32_t *fix_t0;
   float *carry_head;       // eb_*: [ntiles][N] partial of a continued row
   float *carry_tail;       // eb_*: [ntiles][N] partial of a row left open
+  // K, ldb < 2^24 and every byte offset into B fits 32 bits: gathers form
+  // their offsets in 32 bits (load_row_raw<.., true>)
+  int32_t b_off32;
 };
+
+// Whether a K x ldb matrix of elem-byte elements qualifies for 32-bit
+// gather offsets (see KArgs::b_off32).
+inline int32_t b_off32_ok(int32_t K, int64_t ldb, int32_t elem) {
+  const int64_t lim24 = (int64_t)1 << 24;
+  return K < lim24 && ldb < lim24 && ((int64_t)K + 1) * ldb * elem < ((int64_t)1 << 32);
+}
 
 hipError_t launch_rb_sr(int dt, int vec, const KArgs &a, hipStream_t s);
 hipError_t launch_rb_pr(int dt, int vec, const KArgs &a, hipStream_t s);

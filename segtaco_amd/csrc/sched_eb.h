@@ -101,7 +101,7 @@ __device__ __forceinline__ void zero_empty_rows(const KArgs &a, int32_t r0,
 // Otherwise every lookup is an LDS read: a lookup that may go to either
 // address space compiles to a flat load, whose wait drains every gather the
 // lane group has in flight.
-template <int DT, int V, bool kWide>
+template <int DT, int V, bool kWide, bool kOff32>
 __device__ __forceinline__ void eb_sr_walk(const KArgs &a, int32_t blk,
                                            int32_t panel, int32_t *smem) {
   const int lane = threadIdx.x & 63;
@@ -201,7 +201,7 @@ __device__ __forceinline__ void eb_sr_walk(const KArgs &a, int32_t blk,
 #pragma unroll
         for (int u = 0; u < kSrBatch; ++u) {
           const int32_t i = min(b0 + u, cn - 1);
-          buf[u] = load_raw<DT, V>(B, (int64_t)s_crd[i] * ldb + col_ld);
+          buf[u] = load_row_raw<DT, V, kOff32>(B, s_crd[i], ldb, col_ld);
         }
       };
       auto consume = [&](const R (&buf)[kSrBatch], int32_t b0) {
@@ -257,16 +257,24 @@ __device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
   const int32_t t = blk * (kWavesPerBlock << (6 - lq)) + (wave << (6 - lq)) + (lane >> lq);
   const int32_t PW = eb_sr_chunk(lq) + kPosSlack;
   const bool wide = t < a.ntiles && a.tile_row[t + 1] - a.tile_row[t] > PW - 2;
-  if (__ballot(wide))
-    eb_sr_walk<DT, V, true>(a, blk, panel, smem);
-  else
-    eb_sr_walk<DT, V, false>(a, blk, panel, smem);
+  const bool any_wide = __ballot(wide) != 0;
+  if (a.b_off32) {
+    if (any_wide)
+      eb_sr_walk<DT, V, true, true>(a, blk, panel, smem);
+    else
+      eb_sr_walk<DT, V, false, true>(a, blk, panel, smem);
+  } else {
+    if (any_wide)
+      eb_sr_walk<DT, V, true, false>(a, blk, panel, smem);
+    else
+      eb_sr_walk<DT, V, false, false>(a, blk, panel, smem);
+  }
 }
 
 // eb_pr over tile block `blk` (one tile per wave), column panel `panel`;
 // smem holds eb_pr_lds_bytes_fixed() bytes.
-template <int DT, int V>
-__device__ __forceinline__ void eb_pr_body(const KArgs &a, int32_t blk,
+template <int DT, int V, bool kWide, bool kOff32>
+__device__ __forceinline__ void eb_pr_walk(const KArgs &a, int32_t blk,
                                            int32_t panel, int32_t *smem) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -297,9 +305,11 @@ __device__ __forceinline__ void eb_pr_body(const KArgs &a, int32_t blk,
   const int32_t r_hi = min(a.M - 1, a.tile_row[t + 1]);
   int32_t r = a.tile_row[t];
   int32_t rbase = r, pw = 0;
+  // pos[row] from the chunk's LDS window (memory past it only when kWide)
   auto Pos = [&](int32_t row) -> int32_t {
     const int32_t i = row - rbase;
-    return i < pw ? s_pos[i] : pos[row];
+    if constexpr (kWide) return i < pw ? s_pos[i] : pos[row];
+    return s_pos[i];
   };
 
   float open[V];
@@ -337,11 +347,15 @@ __device__ __forceinline__ void eb_pr_body(const KArgs &a, int32_t blk,
       R raw[kPrPasses];
 #pragma unroll
       for (int u = 0; u < kPrPasses; ++u) {
+        // every pass issues its load, with no branch around it (slots past
+        // the chunk re-load its last entry, columns past N column 0), so the
+        // wait before a pass covers only that pass's load
         const int32_t i = pb + u * P + sl;
         const bool ok = i < cn;
         rid[u] = ok ? s_row[i] : INT_MAX;
         vv[u] = ok ? s_val[i] : 0.0f;
-        if (ok && colok) raw[u] = load_raw<DT, V>(B, (int64_t)s_crd[i] * ldb + col);
+        raw[u] = load_row_raw<DT, V, kOff32>(B, s_crd[min(i, cn - 1)], ldb,
+                                             colok ? col : 0);
       }
 #pragma unroll
       for (int u = 0; u < kPrPasses; ++u) {
@@ -398,6 +412,27 @@ __device__ __forceinline__ void eb_pr_body(const KArgs &a, int32_t blk,
     }
     r = s_row[cn - 1];
     wave_sync();
+  }
+}
+
+// The tile's row lookups stay in the LDS window when its rows r0 = tile_row[t]
+// .. r1 = tile_row[t+1] satisfy r1 - r0 <= PW - 2 (as for eb_sr).
+template <int DT, int V>
+__device__ __forceinline__ void eb_pr_body(const KArgs &a, int32_t blk,
+                                           int32_t panel, int32_t *smem) {
+  const int32_t t = blk * kWavesPerBlock + (threadIdx.x >> 6);
+  if (t >= a.ntiles) return;  // uniform over the wave
+  const bool wide = a.tile_row[t + 1] - a.tile_row[t] > kPrChunk + kPosSlack - 2;
+  if (a.b_off32) {
+    if (wide)
+      eb_pr_walk<DT, V, true, true>(a, blk, panel, smem);
+    else
+      eb_pr_walk<DT, V, false, true>(a, blk, panel, smem);
+  } else {
+    if (wide)
+      eb_pr_walk<DT, V, true, false>(a, blk, panel, smem);
+    else
+      eb_pr_walk<DT, V, false, false>(a, blk, panel, smem);
   }
 }
 

@@ -23,7 +23,7 @@
 namespace segtaco {
 
 // One output row of rb_sr for this lane's V columns.
-template <int DT, int V>
+template <int DT, int V, bool kOff32>
 __device__ __forceinline__ void rb_sr_row(const KArgs &a, int32_t row,
                                           int32_t col) {
   const int32_t *__restrict__ crd = a.crd;
@@ -41,7 +41,7 @@ __device__ __forceinline__ void rb_sr_row(const KArgs &a, int32_t row,
     for (; p < e; ++p) {
       float b[V];
       const float v = load_val<DT>(vals, p);
-      load_vec<DT, V>(B, (int64_t)crd[p] * ldb + col, b);
+      load_row_vec<DT, V, kOff32>(B, crd[p], ldb, col, b);
 #pragma unroll
       for (int j = 0; j < V; ++j) neumaier_add(acc[j], cmp[j], v, b[j]);
     }
@@ -59,7 +59,7 @@ __device__ __forceinline__ void rb_sr_row(const KArgs &a, int32_t row,
       for (int u = 0; u < U; ++u) v[u] = load_val<DT>(vals, p + u);
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        load_vec<DT, V>(B, (int64_t)c[u] * ldb + col, b[u]);
+        load_row_vec<DT, V, kOff32>(B, c[u], ldb, col, b[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -68,7 +68,7 @@ __device__ __forceinline__ void rb_sr_row(const KArgs &a, int32_t row,
     for (; p < e; ++p) {
       float b[V];
       const float v = load_val<DT>(vals, p);
-      load_vec<DT, V>(B, (int64_t)crd[p] * ldb + col, b);
+      load_row_vec<DT, V, kOff32>(B, crd[p], ldb, col, b);
 #pragma unroll
       for (int j = 0; j < V; ++j) acc[j] = fmaf(v, b[j], acc[j]);
     }
@@ -78,8 +78,8 @@ __device__ __forceinline__ void rb_sr_row(const KArgs &a, int32_t row,
 
 // rb_sr over row block `blk`, column panel `panel`: lane group gi owns rows
 // [gi*R, gi*R + R), R = 2^lg, one after another.
-template <int DT, int V>
-__device__ __forceinline__ void rb_sr_body(const KArgs &a, int32_t blk,
+template <int DT, int V, bool kOff32>
+__device__ __forceinline__ void rb_sr_walk(const KArgs &a, int32_t blk,
                                            int32_t panel) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -93,13 +93,22 @@ __device__ __forceinline__ void rb_sr_body(const KArgs &a, int32_t blk,
   for (int32_t i = 0; i < R; ++i) {
     const int64_t row = first + i;
     if (row >= a.M) return;
-    rb_sr_row<DT, V>(a, (int32_t)row, col);
+    rb_sr_row<DT, V, kOff32>(a, (int32_t)row, col);
   }
 }
 
-// rb_pr over row block `blk`, column panel `panel`.
 template <int DT, int V>
-__device__ __forceinline__ void rb_pr_body(const KArgs &a, int32_t blk,
+__device__ __forceinline__ void rb_sr_body(const KArgs &a, int32_t blk,
+                                           int32_t panel) {
+  if (a.b_off32)
+    rb_sr_walk<DT, V, true>(a, blk, panel);
+  else
+    rb_sr_walk<DT, V, false>(a, blk, panel);
+}
+
+// rb_pr over row block `blk`, column panel `panel`.
+template <int DT, int V, bool kOff32>
+__device__ __forceinline__ void rb_pr_walk(const KArgs &a, int32_t blk,
                                            int32_t panel) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -129,7 +138,7 @@ __device__ __forceinline__ void rb_pr_body(const KArgs &a, int32_t blk,
       for (; p < e; p += P) {
         float b[V];
         const float v = load_val<DT>(vals, p);
-        load_vec<DT, V>(B, (int64_t)crd[p] * ldb + col, b);
+        load_row_vec<DT, V, kOff32>(B, crd[p], ldb, col, b);
 #pragma unroll
         for (int j = 0; j < V; ++j) neumaier_add(acc[j], cmp[j], v, b[j]);
       }
@@ -147,7 +156,7 @@ __device__ __forceinline__ void rb_pr_body(const KArgs &a, int32_t blk,
         for (int u = 0; u < U; ++u) v[u] = load_val<DT>(vals, p + u * P);
 #pragma unroll
         for (int u = 0; u < U; ++u)
-          load_vec<DT, V>(B, (int64_t)c[u] * ldb + col, b[u]);
+          load_row_vec<DT, V, kOff32>(B, c[u], ldb, col, b[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -156,7 +165,7 @@ __device__ __forceinline__ void rb_pr_body(const KArgs &a, int32_t blk,
       for (; p < e; p += P) {
         float b[V];
         const float v = load_val<DT>(vals, p);
-        load_vec<DT, V>(B, (int64_t)crd[p] * ldb + col, b);
+        load_row_vec<DT, V, kOff32>(B, crd[p], ldb, col, b);
 #pragma unroll
         for (int j = 0; j < V; ++j) acc[j] = fmaf(v, b[j], acc[j]);
       }
@@ -168,6 +177,15 @@ __device__ __forceinline__ void rb_pr_body(const KArgs &a, int32_t blk,
     for (int j = 0; j < V; ++j) acc[j] += __shfl_xor(acc[j], d << lq, 64);
   }
   if (live && slot == 0) store_vec<V>(a.C + (int64_t)row * a.ldc + col, acc);
+}
+
+template <int DT, int V>
+__device__ __forceinline__ void rb_pr_body(const KArgs &a, int32_t blk,
+                                           int32_t panel) {
+  if (a.b_off32)
+    rb_pr_walk<DT, V, true>(a, blk, panel);
+  else
+    rb_pr_walk<DT, V, false>(a, blk, panel);
 }
 
 }  // namespace segtaco

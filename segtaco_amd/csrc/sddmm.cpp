@@ -78,13 +78,7 @@ int32_t segtaco_sddmm(const segtaco_csr_t *S, const void *X, int64_t ldx,
   a.ldy = ldy;
   a.out = out;
   a.lq = lq;
-  {
-    const int64_t lim24 = (int64_t)1 << 24;
-    const int64_t bytes_x = ((int64_t)S->M * ldx + ldx) * eb;
-    const int64_t bytes_y = ((int64_t)S->K * ldy + ldy) * eb;
-    a.off32 = S->M < lim24 && S->K < lim24 && ldx < lim24 && ldy < lim24 &&
-              bytes_x < ((int64_t)1 << 32) && bytes_y < ((int64_t)1 << 32);
-  }
+  a.off32 = b_off32_ok(S->M, ldx, eb) && b_off32_ok(S->K, ldy, eb);
   a.tile = kSdTile;
   a.ntiles = (int32_t)sddmm_tiles(S);
   int32_t *tile_row = static_cast<int32_t *>(workspace);

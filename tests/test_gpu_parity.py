@@ -99,6 +99,11 @@ def _case(name):
         return _mk(rng, np.array([777]), 1000, 64)
     if name == "all_empty_n8":
         return _mk(rng, np.zeros(100, np.int64), 50, 8)
+    if name == "sparse_rows_n8":
+        # one row in ~12 holds entries: every eb tile spans more rows than the
+        # staged row-pointer window (the kernels' out-of-window lookups)
+        lens = np.where(rng.random(20000) < 0.08, rng.integers(1, 4, 20000), 0)
+        return _mk(rng, lens, 3000, 8)
     if name == "novals_n16":
         pos, crd, _, B = _mk(rng, rng.integers(0, 50, 800), 900, 16)
         return pos, crd, None, B
@@ -117,7 +122,8 @@ def _mk(rng, lens, K, N):
 
 CASES = ([f"golden:{p}" for p in GOLDEN] +
          ["uniform_4096_n32", "powerlaw_20000_n128", "empty_runs_n20", "one_long_row_n130",
-          "spmv_n1", "n3_odd", "n512_panels", "single_row", "all_empty_n8", "novals_n16"])
+          "spmv_n1", "n3_odd", "n512_panels", "single_row", "all_empty_n8", "novals_n16",
+          "sparse_rows_n8"])
 CASE_IDS = [c.split("/")[-1] for c in CASES]
 
 
@@ -129,6 +135,25 @@ def test_schedule_parity(case, sched, kw):
     C, C2 = _device_run(pos, crd, vals, B, sched, kw, runs=2)
     _check(pos, crd, vals, B, C, what=f"{case}/{sched}{kw}")
     assert np.array_equal(C, C2), "not deterministic"
+
+
+@pytest.mark.parametrize("sched", ["rb_sr", "rb_pr", "eb_sr", "eb_pr", "eb_cs"])
+def test_b_beyond_32bit_offsets(sched):
+    """K > 2^24 rows of B: gathers take the 64-bit offset path; entries reach
+    the last rows of B."""
+    rng = np.random.default_rng(7)
+    K, N, M = (1 << 24) + 1000, 4, 1500
+    lens = rng.integers(0, 12, M)
+    lens[-1] = 5
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    rows = [np.sort(rng.choice(K, int(l), replace=False)) for l in lens[:-1]]
+    rows.append(np.arange(K - 5, K))  # the last row holds B's last five rows
+    crd = np.concatenate(rows).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(crd)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    print(f"[parity] K={K} N={N} nnz={len(crd)} {sched}", flush=True)
+    (C,) = _device_run(pos, crd, vals, B, sched, {})
+    _check(pos, crd, vals, B, C, what=f"K>2^24/{sched}")
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])

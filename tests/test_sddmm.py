@@ -196,3 +196,26 @@ def test_sddmm_full_size_powerlaw():
     ref = O.sddmm_f64(pos, crd, vals, X, Y)
     scale = O.sddmm_f64(pos, crd, vals, X, Y, absval=True)
     assert _err(out, ref, scale) <= TOL
+
+
+@pytest.mark.gpu
+def test_sddmm_x_beyond_32bit_offsets():
+    """M > 2^24 rows of X: loads take the 64-bit offset path; entries sit in
+    the first and the last rows."""
+    rng = np.random.default_rng(8)
+    M, K, J = (1 << 24) + 300, 500, 4
+    rows = np.sort(np.concatenate([rng.choice(M, 2000, replace=False), [0, M - 1]]))
+    rows = np.unique(rows)
+    lens = np.zeros(M, np.int64)
+    lens[rows] = rng.integers(1, 6, len(rows))
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    crd = np.concatenate([np.sort(rng.choice(K, int(lens[r]), replace=False)) for r in rows]
+                         ).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(crd)).astype(np.float32)
+    X = rng.uniform(-1, 1, (M, J)).astype(np.float32)
+    Y = rng.uniform(-1, 1, (K, J)).astype(np.float32)
+    print(f"[sddmm] M={M} J={J} nnz={len(crd)}", flush=True)
+    out = _run(pos, crd, vals, X, Y)[0]
+    ref = O.sddmm_f64(pos, crd, vals, X, Y)
+    scale = O.sddmm_f64(pos, crd, vals, X, Y, absval=True)
+    assert _err(out, ref, scale) <= TOL
