@@ -167,6 +167,23 @@ VENDOR_ALGS = {0: "default", 1: "csr", 2: "csr_row_split", 3: "csr_nnz_split",
                4: "csr_merge_path"}
 
 
+def spmv_leg(torch, ops, steps, warmup, stream):
+    """SpMV (N = 1) on the headline matrix through the same entry point: each
+    schedule with its default plan, then rocSPARSE's algorithms (compute
+    stage) on the same operands. Compulsory bytes 4(M+1) + 8 nnz +
+    4 K_touched + 4 M."""
+    plans = [(n, S.make_plan(n)) for n in ("rb_sr", "rb_pr", "eb_sr", "eb_pr")]
+    res, cbytes = run_leg(torch, ops, 1, "f32", plans, steps, warmup, stream, 5)
+    out = {k: {"ms": round(v["ms"], 5), "gflops": round(v["gflops"], 2),
+               "hbm_frac": round(v["hbm_frac"], 4), "plan": v["plan"]} for k, v in res.items()}
+    best = min(res, key=lambda k: res[k]["ms"])
+    vend = vendor_leg(torch, ops, 1, steps, stream)
+    ok = [v["gflops"] for v in vend.values() if "gflops" in v]
+    return {"compulsory_bytes": cbytes, "schedules": out, "best": best,
+            "vendor_rocsparse": vend,
+            "vs_rocsparse_best": round(res[best]["gflops"] / max(ok), 3) if ok else None}
+
+
 def vendor_leg(torch, ops, N, reps, stream):
     """rocSPARSE CSR SpMM on the headline operands (compute stage timed with
     HIP events), one line per algorithm; an algorithm that refuses the
@@ -186,10 +203,10 @@ def vendor_leg(torch, ops, N, reps, stream):
             ms = statistics.median(times)
             res[name] = {"ms": round(ms, 5), "ms_min": round(min(times), 5),
                          "gflops": round(flops / (ms * 1e-3) / 1e9, 2)}
-            log(f"rocSPARSE {name}: {ms*1e3:.1f} us  {res[name]['gflops']:.1f} GFLOP/s")
+            log(f"rocSPARSE N={N} {name}: {ms*1e3:.1f} us  {res[name]['gflops']:.1f} GFLOP/s")
         except S.SegtacoError as e:
             res[name] = {"error": str(e)}
-            log(f"rocSPARSE {name}: {e}")
+            log(f"rocSPARSE N={N} {name}: {e}")
     del Bd, C
     torch.cuda.empty_cache()
     return res
@@ -447,9 +464,10 @@ def main():
                          "each 'schedule[:key=value[,key=value]]', e.g. 'eb_sr:tile=256'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline leg only")
-    ap.add_argument("--extra", default="vendor,sddmm,forms,batched",
-                    help="comma-separated legs after the headline: vendor (rocSPARSE), sddmm, "
-                         "forms (N=256 storage types, banded), batched (with serving)")
+    ap.add_argument("--extra", default="vendor,spmv,sddmm,forms,batched",
+                    help="comma-separated legs after the headline: vendor (rocSPARSE), spmv "
+                         "(N = 1), sddmm, forms (N=256 storage types, banded), batched (with "
+                         "serving)")
     ap.add_argument("--batch-matrices", type=int, default=512,
                     help="products in the batching / serving workload (0 skips both)")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
@@ -532,6 +550,8 @@ def main():
         ok = [v["gflops"] for v in vend.values() if "gflops" in v]
         out["vendor_rocsparse"] = vend
         out["vs_rocsparse_best"] = round(out["value"] / max(ok), 3) if ok else None
+    if "spmv" in legs:
+        out["spmv"] = spmv_leg(torch, pl, args.steps, args.warmup, stream)
     if "sddmm" in legs:
         out["sddmm"] = {d: sddmm_leg(torch, pl, 128, d, args.steps, args.warmup, stream)
                         for d in ("f32", "bf16")}
