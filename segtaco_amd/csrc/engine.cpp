@@ -9,6 +9,7 @@
 
 #include "cs.h"
 #include "launch.h"
+#include "spmv.h"
 
 namespace segtaco {
 
@@ -199,6 +200,9 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
         plan->col_lanes = 64 / plan->group;
       if (plan->col_lanes == 0) plan->col_lanes = qn < 32 ? qn : 32;
       if (plan->group == 0) plan->group = 64 / plan->col_lanes;
+      // tiles of 128 entries; at one column (SpMV, k_spmv) that is two
+      // consecutive entries per lane (power-law 262144^2: 32 us per call vs
+      // 34 / 41 / 49 us with 4 / 8 / 16 per lane)
       if (plan->tile == 0) plan->tile = 128;
       break;
     case SEGTACO_EB_CS: {
@@ -425,7 +429,10 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     SEGTACO_HIP(launch_eb_sr(b_dtype, V, a, stream));
   } else {
     a.nblocks = (int32_t)ceil_div(L.ntiles, kWavesPerBlock);
-    SEGTACO_HIP(launch_eb_pr(b_dtype, V, a, stream));
+    if (spmv_applies(a, V))
+      SEGTACO_HIP(launch_spmv(b_dtype, a, stream));
+    else
+      SEGTACO_HIP(launch_eb_pr(b_dtype, V, a, stream));
   }
   SEGTACO_HIP(mark(2));
   SEGTACO_HIP(launch_fixup(V, a, stream));

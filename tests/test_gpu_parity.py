@@ -137,6 +137,47 @@ def test_schedule_parity(case, sched, kw):
     assert np.array_equal(C, C2), "not deterministic"
 
 
+def _spmv_case(name):
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    if name == "powerlaw_20000_n1":
+        pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 2, 20000, 20000)
+        return pos, crd, vals, S.gen_dense(13, 20000, 1)
+    if name == "sparse_rows_n1":  # tiles spanning more rows than the window
+        lens = np.where(rng.random(30000) < 0.03, rng.integers(1, 4, 30000), 0)
+        return _mk(rng, lens, 3000, 1)
+    if name == "long_rows_n1":  # rows spanning many lanes and tiles
+        lens = np.array([0, 7, 20000, 0, 0, 3, 1, 9000, 2, 0, 65, 64, 63, 1], np.int64)
+        return _mk(rng, lens, 25000, 1)
+    if name == "short_rows_n1":  # many rows inside one lane
+        lens = rng.integers(0, 4, 20000)
+        return _mk(rng, lens, 5000, 1)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("tile", [64, 192, 512, 1024])
+@pytest.mark.parametrize("case", ["powerlaw_20000_n1", "sparse_rows_n1", "long_rows_n1",
+                                  "short_rows_n1"])
+def test_spmv_kernel_parity(case, tile):
+    """eb_pr at N = 1 with tiles of 64 * E entries runs the SpMV kernel (E
+    consecutive entries per lane, one segmented scan over the wave)."""
+    pos, crd, vals, B = _spmv_case(case)
+    print(f"[spmv] {case} tile {tile} M={len(pos)-1} nnz={len(crd)}", flush=True)
+    C, C2 = _device_run(pos, crd, vals, B, "eb_pr", dict(tile=tile), runs=2)
+    _check(pos, crd, vals, B, C, what=f"spmv {case}/tile {tile}")
+    assert np.array_equal(C, C2), "not deterministic"
+
+
+def test_spmv_kernel_strided_vectors():
+    """x and y with leading dimensions 3 and 2 (every third / second element)."""
+    pos, crd, vals, B = _spmv_case("short_rows_n1")
+    K = B.shape[0]
+    A = DeviceCSR(pos, crd, vals, K)
+    Bs = np.zeros((K, 3), np.float32)
+    Bs[:, 0] = B[:, 0]
+    C = run_spmm(A, dense_dev(Bs), 1, S.make_plan("eb_pr"), ldb=3, ldc=2)
+    _check(pos, crd, vals, B, C[: A.M, :1].cpu().numpy(), what="spmv strided")
+
+
 @pytest.mark.parametrize("sched", ["rb_sr", "rb_pr", "eb_sr", "eb_pr", "eb_cs"])
 def test_b_beyond_32bit_offsets(sched):
     """K > 2^24 rows of B: gathers take the 64-bit offset path; entries reach

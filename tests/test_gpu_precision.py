@@ -54,6 +54,9 @@ def _cases():
     yield "longrows_n40", _mk(rng, lens, 4000, 40)
     yield "n130", _mk(rng, rng.integers(0, 20, 700), 900, 130)
     yield "n7", _mk(rng, rng.integers(0, 20, 700), 900, 7)
+    lens = rng.integers(0, 30, 3000)
+    lens[[7, 2000]] = [4000, 1500]
+    yield "spmv_n1", _mk(rng, lens, 5000, 1)
 
 
 CASES = list(_cases())
