@@ -566,7 +566,8 @@ def main():
         torch.cuda.empty_cache()
         banded = Operands(torch, W["banded"])
         res, _ = run_leg(torch, banded, 256, "bf16", [("eb_sr", S.make_plan("eb_sr")),
-                                                     ("rb_sr", S.make_plan("rb_sr"))],
+                                                     ("rb_sr", S.make_plan("rb_sr")),
+                                                     ("bd", S.make_plan("bd"))],
                          max(5, args.steps // 5), 2, stream, 10)
         forms["banded_1048576_n256_bf16"] = best_form(res)
         del banded

@@ -195,6 +195,12 @@ void spmm_csr_gpu_cusparse(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B)
  * the rows of a unit of tiles accumulate in LDS. Same results as eb_sr with
  * the same tile for column-sorted rows; no compensated summation. */
 #define SEGTACO_EB_CS 4
+/* Block-dense, for BF16 / FP16 storage: rows in blocks of 32 (tile = 32);
+ * a block whose entries span at most 256 columns is scattered into a dense
+ * tile and multiplied with the matrix cores (MFMA, FP32 accumulation),
+ * wider blocks run a per-row vector walk. For banded and block-structured
+ * A. Runs standalone only (not in batches or the server). */
+#define SEGTACO_BD 5
 
 /* A schedule and its parameters. Zero fields are resolved by
  * segtaco_plan_resolve from the operands. */

@@ -27,8 +27,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SEGTACO_LIB") or os.path.join(_HERE, "libsegtaco.so")
 
 # Schedules / dtypes / status codes (include/segtaco/spmm.h)
-RB_SR, RB_PR, EB_SR, EB_PR, EB_CS = 0, 1, 2, 3, 4
-SCHEDULES = {"rb_sr": RB_SR, "rb_pr": RB_PR, "eb_sr": EB_SR, "eb_pr": EB_PR, "eb_cs": EB_CS}
+RB_SR, RB_PR, EB_SR, EB_PR, EB_CS, BD = 0, 1, 2, 3, 4, 5
+SCHEDULES = {"rb_sr": RB_SR, "rb_pr": RB_PR, "eb_sr": EB_SR, "eb_pr": EB_PR, "eb_cs": EB_CS,
+             "bd": BD}
 DT_F32, DT_BF16, DT_F16 = 0, 1, 2
 DTYPES = {"f32": DT_F32, "bf16": DT_BF16, "fp16": DT_F16}
 OK = 0

@@ -179,6 +179,10 @@ int32_t segtaco_tune_cache_get(uint64_t sig, segtaco_plan_t *out) {
 
 int32_t segtaco_tune_cache_put(uint64_t sig, const segtaco_plan_t *plan) {
   if (!plan) return set_error(SEGTACO_ERR_INVALID, "null plan");
+  // the cache feeds batches and the server, which run rb_sr .. eb_pr
+  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_EB_PR)
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "tune cache: schedule %d runs standalone only",
+                     plan->schedule);
   std::lock_guard<std::mutex> lock(g_cache_mu);
   cache()[sig] = *plan;
   return SEGTACO_OK;

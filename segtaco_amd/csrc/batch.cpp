@@ -186,9 +186,10 @@ int32_t segtaco_batch_create(const segtaco_batch_item_t *items, int32_t n,
       b->fix_prefix[i + 1] = b->fix_prefix[i];
       continue;
     }
-    if (plan.schedule == SEGTACO_EB_CS) {
-      // its unit tables and LDS sizing assume one standalone launch
-      st = set_error(SEGTACO_ERR_UNSUPPORTED, "batch item %d: eb_cs runs standalone only", i);
+    if (plan.schedule == SEGTACO_EB_CS || plan.schedule == SEGTACO_BD) {
+      // their tables / tiles and LDS sizing assume one standalone launch
+      st = set_error(SEGTACO_ERR_UNSUPPORTED, "batch item %d: %s runs standalone only", i,
+                     plan.schedule == SEGTACO_EB_CS ? "eb_cs" : "bd");
       break;
     }
     // Alignment-aware resolution (vec from the real pointers).

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <vector>
 
+#include "bd.h"
 #include "cs.h"
 #include "launch.h"
 #include "spmv.h"
@@ -145,7 +146,7 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
                 int64_t ldb, int64_t ldc, const void *B, const void *C,
                 segtaco_plan_t *plan) {
   if (!A || !plan) return set_error(SEGTACO_ERR_INVALID, "null argument");
-  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_EB_CS)
+  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_BD)
     return set_error(SEGTACO_ERR_INVALID, "unknown schedule %d",
                      plan->schedule);
   if (b_dtype < SEGTACO_DT_F32 || b_dtype > SEGTACO_DT_F16)
@@ -231,6 +232,19 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
                          "eb_cs: compensated summation is not supported");
       break;
     }
+    case SEGTACO_BD:
+      // 32-row blocks on the matrix cores; lanes and groups are fixed
+      if (b_dtype == SEGTACO_DT_F32)
+        return set_error(SEGTACO_ERR_UNSUPPORTED, "bd: BF16 / FP16 storage only");
+      if (plan->tile == 0) plan->tile = kBdRows;
+      if (plan->tile != kBdRows)
+        return set_error(SEGTACO_ERR_INVALID, "bd: tile (rows per block) must be %d",
+                         kBdRows);
+      if (plan->col_lanes == 0) plan->col_lanes = 1;
+      if (plan->group == 0) plan->group = 1;
+      if (plan->col_lanes != 1 || plan->group != 1)
+        return set_error(SEGTACO_ERR_INVALID, "bd: col_lanes and group must be 1");
+      break;
   }
   const int32_t Q = plan->col_lanes, G = plan->group;
   if (!is_pow2(Q) || Q > 64)
@@ -342,6 +356,16 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
       SEGTACO_HIP(mark(0));
       SEGTACO_HIP(mark(1));
       SEGTACO_HIP(launch_rb_pr(b_dtype, V, a, stream));
+      SEGTACO_HIP(mark(2));
+      SEGTACO_HIP(mark(3));
+      return SEGTACO_OK;
+    }
+    case SEGTACO_BD: {
+      a.nblocks = (int32_t)ceil_div(A->M, (int64_t)kBdRows);
+      a.panels = (int32_t)ceil_div(N, (int64_t)bd_col_tiles(N) * kWavesPerBlock * 32);
+      SEGTACO_HIP(mark(0));
+      SEGTACO_HIP(mark(1));
+      SEGTACO_HIP(launch_bd(b_dtype, a, stream));
       SEGTACO_HIP(mark(2));
       SEGTACO_HIP(mark(3));
       return SEGTACO_OK;

@@ -176,3 +176,24 @@ def test_eb_cs_plan_resolution():
     # small matrices: the tile floor
     p = S.plan_resolve(_desc(M=100, K=80, nnz=1000), 64, S.DT_F32, S.make_plan("eb_cs"))
     assert p.tile == 32
+
+
+def test_bd_plan_resolution():
+    """bd: 16-bit storage only, 32-row blocks, lanes and groups fixed at 1."""
+    B16 = _desc(M=1000, K=1000, nnz=20000, dtype=S.DT_BF16)
+    p = S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("bd"), 256, 256)
+    assert (p.tile, p.col_lanes, p.group, p.comp_threshold) == (32, 1, 1, 1024)
+    assert S.workspace_bytes(B16, 256, p) == 0
+    with pytest.raises(S.SegtacoError):  # FP32 has no faster matrix-core path here
+        S.plan_resolve(_desc(M=1000, K=1000, nnz=20000), 64, S.DT_F32, S.make_plan("bd"))
+    for bad in (dict(tile=64), dict(col_lanes=4), dict(group=2)):
+        with pytest.raises(S.SegtacoError):
+            S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("bd", **bad), 256, 256)
+
+
+def test_tune_cache_takes_batch_schedules_only():
+    """The plan cache feeds batches and the server: standalone-only schedules
+    are refused."""
+    for name in ("eb_cs", "bd"):
+        with pytest.raises(S.SegtacoError):
+            S.tune_cache_put(0x1234, S.make_plan(name))

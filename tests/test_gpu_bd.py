@@ -1,0 +1,120 @@
+"""GPU parity of the block-dense schedule (bd): BF16 / FP16 storage, 32-row
+blocks on the matrix cores when their columns span at most 256, a per-row
+vector walk otherwise.
+
+Tolerances (as for every 16-bit path):
+  * vs FP64 of the *rounded* inputs, oracle scaled metric: <= 1e-5;
+  * vs FP64 of the original FP32 inputs, norm-wise: <= 1e-2.
+Two runs are bit-identical (no atomics).
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import segtaco_amd as S
+from gpu_util import DeviceCSR, dense_dev, run_spmm
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(rng, M, K, spans):
+    """CSR whose row i holds sorted unique columns drawn from spans[i] =
+    (lo, hi) (half-open, clipped to K) with the given density."""
+    pos, crd = [0], []
+    for lo, hi, dens in spans:
+        lo, hi = max(0, lo), min(K, hi)
+        n = int(round((hi - lo) * dens)) if hi > lo else 0
+        cols = np.sort(rng.choice(np.arange(lo, hi), n, replace=False)) if n else []
+        crd.extend(int(c) for c in cols)
+        pos.append(len(crd))
+    pos = np.array(pos, np.int32)
+    crd = np.array(crd, np.int32)
+    vals = rng.uniform(-1, 1, len(crd)).astype(np.float32)
+    return pos, crd, vals
+
+
+def _case(name):
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    if name == "banded_bw32":  # |i - j| <= 32, every entry stored: windows of 96
+        pos, crd, vals = S.gen_csr(S.GEN_BANDED, 3, 3000, 3000, bw=32)
+        return pos, crd, vals, 3000
+    if name == "banded_ragged":  # M not a multiple of 32, empty rows and blocks
+        M = K = 2011
+        spans = [(i - 20, i + 21, 0.6) for i in range(M)]
+        for i in list(range(100, 180)) + [5, 6, 2010]:
+            spans[i] = (0, 0, 0.0)
+        return (*_rows(rng, M, K, spans), K)
+    if name == "window_edges":  # blocks of width 256 (matrix cores) and 257 (vector)
+        M, K = 32 * 6, 2000
+        spans = []
+        for b, w in enumerate([256, 257, 200, 256, 255, 400]):
+            c0 = 150 * b
+            for i in range(32):
+                spans.append((c0, c0 + w, 0.25))
+            spans[32 * b] = (c0, c0 + 1, 1.0)  # pin the window's first column
+            spans[32 * b + 31] = (c0 + w - 1, c0 + w, 1.0)  # and its last
+        return (*_rows(rng, M, K, spans), K)
+    if name == "powerlaw_fallback":  # nothing dense: all blocks on the vector path
+        pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 4, 3000, 3000)
+        return pos, crd, vals, 3000
+    if name == "long_rows_fallback":  # rows above the compensation threshold
+        M, K = 64, 6000
+        spans = [(0, 6000, 0.5) if i in (3, 40) else (i * 10, i * 10 + 30, 0.5)
+                 for i in range(M)]
+        return (*_rows(rng, M, K, spans), K)
+    if name == "near_k_edge":  # windows ending at K: staged rows past K are zeros
+        M, K = 96, 100
+        spans = [(max(0, K - 70 + (i % 7)), K, 0.5) for i in range(M)]
+        return (*_rows(rng, M, K, spans), K)
+    raise KeyError(name)
+
+
+CASES = ["banded_bw32", "banded_ragged", "window_edges", "powerlaw_fallback",
+         "long_rows_fallback", "near_k_edge"]
+
+
+def _run(pos, crd, vals, B, dtype, ldb=None, ldc=None, runs=1):
+    K, N = B.shape
+    A = DeviceCSR(pos, crd, vals, K, dtype=dtype)
+    ldb = ldb or N
+    Bp = np.zeros((K, ldb), np.float32)
+    Bp[:, :N] = B
+    outs = []
+    for _ in range(runs):
+        C = run_spmm(A, dense_dev(Bp, dtype), N, S.make_plan("bd"), ldb=ldb, ldc=ldc,
+                     b_dtype=dtype)
+        outs.append(C[: A.M, :N].cpu().numpy())
+    return outs
+
+
+def _check(pos, crd, vals, B, C, dtype, what):
+    vr = None if vals is None else O.round_to(dtype, vals)[1]
+    Br = O.round_to(dtype, B)[1]
+    err_r = O.scaled_err(C, O.spmm_f64(pos, crd, vr, Br), O.spmm_absbound(pos, crd, vr, Br))
+    assert err_r <= 1e-5, f"{what}: vs FP64 of rounded inputs {err_r:.3e}"
+    fro, _ = O.rel_norm_errs(C, O.spmm_f64(pos, crd, vals, B))
+    assert fro <= 1e-2, f"{what}: vs FP64 of FP32 inputs {fro:.3e}"
+
+
+@pytest.mark.parametrize("N", [256, 128, 40, 7, 300])
+@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+@pytest.mark.parametrize("case", CASES)
+def test_bd_parity(case, dtype, N):
+    pos, crd, vals, K = _case(case)
+    B = np.random.default_rng(N).uniform(-1, 1, (K, N)).astype(np.float32)
+    print(f"[bd] {case} {dtype} N={N} M={len(pos)-1} nnz={len(crd)}", flush=True)
+    C, C2 = _run(pos, crd, vals, B, dtype, runs=2)
+    _check(pos, crd, vals, B, C, dtype, f"{case}/{dtype}/N={N}")
+    assert np.array_equal(C.view(np.uint32), C2.view(np.uint32)), "not deterministic"
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+def test_bd_strided_unaligned_and_novals(dtype):
+    """ldb = 261 (no 16-byte B loads), ldc = 300, and A without values."""
+    pos, crd, _, K = _case("banded_bw32")
+    N = 256
+    B = np.random.default_rng(9).uniform(-1, 1, (K, N)).astype(np.float32)
+    (C,) = _run(pos, crd, None, B, dtype, ldb=261, ldc=300)
+    _check(pos, crd, None, B, C, dtype, f"strided/{dtype}")
