@@ -1,5 +1,6 @@
 // engine.cpp — the device-resident SpMM API: plan resolution, workspace
-// layout, validation and kernel sequencing for the four schedules.
+// layout, validation and kernel sequencing for every schedule (the
+// reference's four, eb_cs and bd).
 #include "engine.h"
 
 #include <climits>
