@@ -165,6 +165,7 @@ def sddmm_leg(torch, ops, J, dtype, steps, warmup, stream):
 
 VENDOR_ALGS = {0: "default", 1: "csr", 2: "csr_row_split", 3: "csr_nnz_split",
                4: "csr_merge_path"}
+VENDOR_SPMV_ALGS = {0: "default", 1: "csr_adaptive", 2: "csr_rowsplit", 3: "csr_lrb"}
 
 
 def spmv_leg(torch, ops, steps, warmup, stream):
@@ -178,6 +179,24 @@ def spmv_leg(torch, ops, steps, warmup, stream):
                "hbm_frac": round(v["hbm_frac"], 4), "plan": v["plan"]} for k, v in res.items()}
     best = min(res, key=lambda k: res[k]["ms"])
     vend = vendor_leg(torch, ops, 1, steps, stream)
+    # rocSPARSE's dedicated SpMV (compute stage; its analysis is excluded)
+    x = ops.dense(5, 1, "f32")
+    y = torch.empty(ops.M, dtype=torch.float32, device="cuda")
+    A = ops.desc("f32")
+    for alg, name in VENDOR_SPMV_ALGS.items():
+        try:
+            for _ in range(2):
+                S.vendor_spmv(A, x.data_ptr(), y.data_ptr(), alg, stream.cuda_stream, timed=True)
+            times = [S.vendor_spmv(A, x.data_ptr(), y.data_ptr(), alg, stream.cuda_stream,
+                                   timed=True) for _ in range(max(steps, 1))]
+            ms = statistics.median(times)
+            vend[f"spmv_{name}"] = {"ms": round(ms, 5), "ms_min": round(min(times), 5),
+                                    "gflops": round(2.0 * ops.nnz / (ms * 1e-3) / 1e9, 2)}
+            log(f"rocSPARSE SpMV {name}: {ms*1e3:.1f} us")
+        except S.SegtacoError as e:
+            vend[f"spmv_{name}"] = {"error": str(e)}
+            log(f"rocSPARSE SpMV {name}: {e}")
+    del x, y
     ok = [v["gflops"] for v in vend.values() if "gflops" in v]
     return {"compulsory_bytes": cbytes, "schedules": out, "best": best,
             "vendor_rocsparse": vend,

@@ -288,6 +288,15 @@ int32_t segtaco_vendor_spmm(const segtaco_csr_t *A, const float *B, int64_t ldb,
                             int32_t N, float *C, int64_t ldc, int32_t alg,
                             void *stream, float *ms_out);
 
+/* rocSPARSE CSR SpMV, y = A x (FP32, contiguous x and y): the comparison
+ * row for the N = 1 case (the reference's vendor row is cusparseSpMM,
+ * spmm_csr_gpu_cusparse.h:171-260; rocSPARSE's dedicated SpMV is the fairer
+ * opponent at one column). alg: 0 default, 1 csr_adaptive, 2 csr_rowsplit,
+ * 3 csr_lrb, 4 csr_nnzsplit (which this ROCm release's generic SpMV
+ * refuses for FP32 CSR). Timing as for segtaco_vendor_spmm. */
+int32_t segtaco_vendor_spmv(const segtaco_csr_t *A, const float *x, float *y,
+                            int32_t alg, void *stream, float *ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -298,6 +298,9 @@ def lib() -> ctypes.CDLL:
     L.segtaco_sddmm.restype = ctypes.c_int32
     L.segtaco_sddmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int64, _vp, ctypes.c_int64,
                                 ctypes.c_int32, ctypes.c_int32, _vp, _vp, ctypes.c_size_t, _vp]
+    L.segtaco_vendor_spmv.restype = ctypes.c_int32
+    L.segtaco_vendor_spmv.argtypes = [ctypes.POINTER(CSR), _vp, _vp, ctypes.c_int32, _vp,
+                                      ctypes.POINTER(ctypes.c_float)]
     L.segtaco_vendor_spmm.restype = ctypes.c_int32
     L.segtaco_vendor_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int64, ctypes.c_int32,
                                       _vp, ctypes.c_int64, ctypes.c_int32, _vp, _f32p]
@@ -533,6 +536,16 @@ def compare_matrices(a: np.ndarray, b: np.ndarray) -> float:
     ta = _taco_dense(np.ascontiguousarray(a, dtype=np.float32), keep)
     tb = _taco_dense(np.ascontiguousarray(b, dtype=np.float32), keep)
     return float(lib().segtaco_compare_matrices(ctypes.byref(ta), ctypes.byref(tb)))
+
+
+def vendor_spmv(A: "CSR", x_ptr: int, y_ptr: int, alg: int = 0, stream: int | None = None,
+                timed: bool = False):
+    """rocSPARSE CSR SpMV y = A x on device operands (comparison row). Returns
+    the compute-stage ms when timed (synchronous), else None (asynchronous)."""
+    ms = ctypes.c_float(0.0)
+    check(lib().segtaco_vendor_spmv(ctypes.byref(A), x_ptr, y_ptr, alg, stream or None,
+                                    ctypes.byref(ms) if timed else None))
+    return float(ms.value) if timed else None
 
 
 def vendor_spmm(A: "CSR", B_ptr: int, ldb: int, N: int, C_ptr: int, ldc: int, alg: int = 0,

@@ -115,7 +115,100 @@ int32_t vendor_spmm(const segtaco_csr_t *A, const float *B, int64_t ldb,
   return SEGTACO_OK;
 }
 
+rocsparse_spmv_alg spmv_alg_of(int32_t alg) {
+  switch (alg) {
+    case 1: return rocsparse_spmv_alg_csr_adaptive;
+    case 2: return rocsparse_spmv_alg_csr_rowsplit;
+    case 3: return rocsparse_spmv_alg_csr_lrb;
+    case 4: return rocsparse_spmv_alg_csr_nnzsplit;
+    default: return rocsparse_spmv_alg_default;
+  }
+}
+
+// y = A x with rocSPARSE's SpMV (FP32). ev0/ev1 (may be null) bracket the
+// compute stage on `stream`; the preprocessing stage (adaptive / LRB
+// analysis of A) runs before ev0, as the SpMM row's does.
+int32_t vendor_spmv(const segtaco_csr_t *A, const float *x, float *y, int32_t alg,
+                    hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
+  if (!A || A->M < 0 || A->K < 0 || A->nnz < 0 || !x || !y)
+    return set_error(SEGTACO_ERR_INVALID, "vendor spmv: bad operand");
+  if (A->vals_dtype != SEGTACO_DT_F32 || (A->nnz > 0 && !A->vals))
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "vendor spmv: needs FP32 values");
+  VendorState &s = vstate();
+  std::lock_guard<std::mutex> lock(s.mu);
+  if (!s.handle) SEGTACO_SPARSE(rocsparse_create_handle(&s.handle));
+  SEGTACO_SPARSE(rocsparse_set_stream(s.handle, stream));
+  rocsparse_spmat_descr mat = nullptr;
+  rocsparse_dnvec_descr vx = nullptr, vy = nullptr;
+  struct Release {
+    rocsparse_spmat_descr &m;
+    rocsparse_dnvec_descr &a, &b;
+    ~Release() {
+      if (m) rocsparse_destroy_spmat_descr(m);
+      if (a) rocsparse_destroy_dnvec_descr(a);
+      if (b) rocsparse_destroy_dnvec_descr(b);
+    }
+  } release{mat, vx, vy};
+  SEGTACO_SPARSE(rocsparse_create_csr_descr(
+      &mat, A->M, A->K, A->nnz, const_cast<int32_t *>(A->pos),
+      const_cast<int32_t *>(A->crd), const_cast<void *>(A->vals),
+      rocsparse_indextype_i32, rocsparse_indextype_i32, rocsparse_index_base_zero,
+      rocsparse_datatype_f32_r));
+  SEGTACO_SPARSE(rocsparse_create_dnvec_descr(&vx, A->K, const_cast<float *>(x),
+                                              rocsparse_datatype_f32_r));
+  SEGTACO_SPARSE(rocsparse_create_dnvec_descr(&vy, A->M, y, rocsparse_datatype_f32_r));
+  const float alpha = 1.0f, beta = 0.0f;
+  const rocsparse_spmv_alg a = spmv_alg_of(alg);
+  size_t need = 0;
+  // the generic SpMV entry point of this ROCm release (its successor takes
+  // a separate descriptor object; the computation is the same)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+  auto spmv = [&](rocsparse_spmv_stage stage, void *buf) {
+    return rocsparse_spmv(s.handle, rocsparse_operation_none, &alpha, mat, vx, &beta, vy,
+                          rocsparse_datatype_f32_r, a, stage, &need, buf);
+  };
+#pragma clang diagnostic pop
+  SEGTACO_SPARSE(spmv(rocsparse_spmv_stage_buffer_size, nullptr));
+  if (need > s.buf_bytes) {
+    if (s.buf) {
+      SEGTACO_HIP(hipStreamSynchronize(stream));
+      SEGTACO_HIP(hipFree(s.buf));
+      s.buf = nullptr;
+      s.buf_bytes = 0;
+    }
+    SEGTACO_HIP(hipMalloc(&s.buf, need > 0 ? need : 1));
+    s.buf_bytes = need > 0 ? need : 1;
+  }
+  SEGTACO_SPARSE(spmv(rocsparse_spmv_stage_preprocess, s.buf));
+  // y cleared first (outside the timed stage): with beta = 0 some of the
+  // algorithms still read y, and 0 * NaN left there would survive
+  if (A->M > 0) SEGTACO_HIP(hipMemsetAsync(y, 0, sizeof(float) * (size_t)A->M, stream));
+  if (ev0) SEGTACO_HIP(hipEventRecord(ev0, stream));
+  SEGTACO_SPARSE(spmv(rocsparse_spmv_stage_compute, s.buf));
+  if (ev1) SEGTACO_HIP(hipEventRecord(ev1, stream));
+  return SEGTACO_OK;
+}
+
 }  // namespace segtaco
+
+extern "C" int32_t segtaco_vendor_spmv(const segtaco_csr_t *A, const float *x, float *y,
+                                       int32_t alg, void *stream, float *ms_out) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!ms_out) return segtaco::vendor_spmv(A, x, y, alg, s, nullptr, nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  SEGTACO_HIP(hipEventCreate(&e0));
+  SEGTACO_HIP(hipEventCreate(&e1));
+  int32_t st = segtaco::vendor_spmv(A, x, y, alg, s, e0, e1);
+  if (st == SEGTACO_OK) {
+    if (hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms_out, e0, e1) != hipSuccess)
+      st = segtaco::set_error(SEGTACO_ERR_HIP, "segtaco_vendor_spmv: event timing");
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
 
 extern "C" int32_t segtaco_vendor_spmm(const segtaco_csr_t *A, const float *B,
                                        int64_t ldb, int32_t N, float *C,

@@ -269,6 +269,20 @@ def test_vendor_row_matches_oracle(case):
     _check(pos, crd, vals, B, C.cpu().numpy(), what=f"rocsparse {case}")
 
 
+@pytest.mark.parametrize("alg", [0, 1, 2, 3])
+def test_vendor_spmv_matches_oracle(alg):
+    """rocSPARSE's SpMV (the N = 1 comparison row) computes the same product."""
+    t = torch()
+    pos, crd, vals, B = _spmv_case("powerlaw_20000_n1")
+    A = DeviceCSR(pos, crd, vals, B.shape[0])
+    xd = dense_dev(B)
+    y = t.full((A.M, 1), float("nan"), dtype=t.float32, device="cuda")
+    ms = S.vendor_spmv(A.desc, xd.data_ptr(), y.data_ptr(), alg, stream_ptr(), timed=True)
+    t.cuda.synchronize()
+    assert ms > 0
+    _check(pos, crd, vals, B, y.cpu().numpy(), what=f"rocsparse spmv alg {alg}")
+
+
 @pytest.mark.parametrize("sched", ["rb_sr", "eb_sr", "eb_pr"])
 def test_stage_times_run_the_same_product(sched):
     """segtaco_spmm_stage_times computes the same C as segtaco_spmm and reports
