@@ -289,21 +289,29 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
       constexpr int32_t CK = Q * Q;
       int32_t mc[Q], ms[Q], nc[Q], ns[Q];
       float mv[Q], nv[Q];
+      // Every load below is unconditional (entries past the tile re-read its
+      // last entry, columns past N column 0, neither used), so the number of
+      // loads in flight is the same on every path and each wait covers only
+      // the batch it needs. Called only while s < e.
+      const int32_t col_ld = colok ? col : 0;
       auto load_meta = [&](int32_t c0, int32_t (&c)[Q], float (&v)[Q], int32_t (&sl)[Q]) {
 #pragma unroll
         for (int32_t k = 0; k < Q; ++k) {
           const int32_t jj = c0 + Q * k + q;
+          const int32_t jc = min(jj, e - 1);
           const bool ok = jj < e;
-          c[k] = ok ? a.s_col[jj] : 0;
-          v[k] = ok ? load_val<DT>(a.s_val, jj) : 0.0f;
-          sl[k] = ok ? (int32_t)a.s_slot[jj] : -1;
+          c[k] = a.s_col[jc];
+          const float vk = load_val<DT>(a.s_val, jc);
+          const int32_t sk = (int32_t)a.s_slot[jc];
+          v[k] = ok ? vk : 0.0f;
+          sl[k] = ok ? sk : -1;
         }
       };
-      auto issue = [&](R (&buf)[Q], int32_t cq, int32_t j0) {
+      auto issue = [&](R (&buf)[Q], int32_t cq, int32_t) {
 #pragma unroll
         for (int32_t u = 0; u < Q; ++u) {
           const int32_t cu = __shfl(cq, gbase + u, 64);
-          if (j0 + u < e && colok) buf[u] = load_raw<DT, V>(a.B, (int64_t)cu * a.ldb + col);
+          buf[u] = load_raw<DT, V>(a.B, (int64_t)cu * a.ldb + col_ld);
         }
       };
       auto consume = [&](const R (&buf)[Q], float vq, int32_t sq, int32_t j0) {
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
       }
       for (int32_t c0 = s; c0 < e; c0 += CK) {
         const bool more = c0 + CK < e;
-        if (more) load_meta(c0 + CK, nc, nv, ns);
+        load_meta(c0 + CK, nc, nv, ns);  // past the tile: clamped, unused
 #pragma unroll
         for (int32_t k = 0; k < Q; ++k) {
           const int32_t j0 = c0 + Q * k;
@@ -350,7 +358,7 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
               issue(buf0, mc[k + 1], j0 + Q);
             else
               issue(buf1, mc[k + 1], j0 + Q);
-          } else if (more) {
+          } else {
             issue(buf0, nc[0], c0 + CK);  // Q is even: batch 0 uses buf0
           }
           if (k & 1)
