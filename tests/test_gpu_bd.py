@@ -118,3 +118,16 @@ def test_bd_strided_unaligned_and_novals(dtype):
     B = np.random.default_rng(9).uniform(-1, 1, (K, N)).astype(np.float32)
     (C,) = _run(pos, crd, None, B, dtype, ldb=261, ldc=300)
     _check(pos, crd, None, B, C, dtype, f"strided/{dtype}")
+
+
+def test_batch_refuses_bd():
+    """bd runs standalone only: a batch item with a bd plan is refused."""
+    from gpu_util import torch
+    t = torch()
+    pos, crd, vals = S.gen_csr(S.GEN_BANDED, 1, 500, 500, bw=8)
+    A = DeviceCSR(pos, crd, vals, 500, dtype="bf16")
+    B = dense_dev(S.gen_dense(2, 500, 16), "bf16")
+    C = t.empty((500, 16), dtype=t.float32, device="cuda")
+    item = S.batch_item(A.desc, B.data_ptr(), 16, C.data_ptr(), 16, 16, S.make_plan("bd"))
+    with pytest.raises(S.SegtacoError, match="standalone"):
+        S.Batch([item], dtype="bf16")
