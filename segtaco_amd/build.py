@@ -35,7 +35,7 @@ COMMON = [
 LIB_SOURCES = [
     "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
     "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp", "io.cpp", "vendor.cpp",
-    "k_sddmm.hip", "sddmm.cpp", "k_cs.hip", "k_spmv.hip", "k_bd.hip",
+    "k_sddmm.hip", "sddmm.cpp", "k_cs.hip", "k_spmv.hip", "k_bd.hip", "staging.cpp",
 ]
 # rocSPARSE backs the vendor comparison row only (vendor.cpp).
 LIB_LINK = ["-L/opt/rocm/lib", "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"]

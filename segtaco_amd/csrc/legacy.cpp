@@ -11,13 +11,16 @@
 //     (pos monotone, crd in [0, K)), so a malformed operand is an error
 //     rather than an out-of-bounds device access;
 //   * errors exit (as gpuErrchk does, gpu_library.h:22-30) only in the default
-//     error mode; SEGTACO_ERRMODE_RECORD returns with segtaco_last_error set.
+//     error mode; SEGTACO_ERRMODE_RECORD returns with segtaco_last_error set;
+//   * large operands travel through a ring of pinned buffers (staging.h)
+//     instead of pageable copies, about twice the PCIe rate.
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
 
 #include "engine.h"
 #include "legacy_table.h"
+#include "staging.h"
 
 namespace segtaco {
 
@@ -58,6 +61,7 @@ struct LegacyState {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   char *pool = nullptr;
   size_t pool_bytes = 0;
+  Stager stager;  // used under mu only
 };
 
 LegacyState &state() {
@@ -160,13 +164,10 @@ int32_t run_legacy(int id, taco_tensor_t *C, taco_tensor_t *A,
                o_B = take(b_B), o_C = take(b_C), o_ws = take(b_ws);
   SEGTACO_RET(ensure_pool(s, off));
   char *base = s.pool;
-  SEGTACO_HIP(hipMemcpyAsync(base + o_pos, pos, b_pos, hipMemcpyHostToDevice, s.stream));
-  if (b_crd)
-    SEGTACO_HIP(hipMemcpyAsync(base + o_crd, crd, b_crd, hipMemcpyHostToDevice, s.stream));
-  if (b_val)
-    SEGTACO_HIP(hipMemcpyAsync(base + o_val, A->vals, b_val, hipMemcpyHostToDevice, s.stream));
-  if (b_B && B->vals)
-    SEGTACO_HIP(hipMemcpyAsync(base + o_B, B->vals, b_B, hipMemcpyHostToDevice, s.stream));
+  SEGTACO_RET(s.stager.h2d(base + o_pos, pos, b_pos, s.stream));
+  if (b_crd) SEGTACO_RET(s.stager.h2d(base + o_crd, crd, b_crd, s.stream));
+  if (b_val) SEGTACO_RET(s.stager.h2d(base + o_val, A->vals, b_val, s.stream));
+  if (b_B && B->vals) SEGTACO_RET(s.stager.h2d(base + o_B, B->vals, b_B, s.stream));
   dA.pos = reinterpret_cast<const int32_t *>(base + o_pos);
   dA.crd = reinterpret_cast<const int32_t *>(base + o_crd);
   dA.vals = b_val ? base + o_val : nullptr;
@@ -187,8 +188,7 @@ int32_t run_legacy(int id, taco_tensor_t *C, taco_tensor_t *A,
   float ms = -1.0f;
   SEGTACO_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
   g_timer_ms = ms;
-  SEGTACO_HIP(hipMemcpyAsync(C->vals, base + o_C, b_C, hipMemcpyDeviceToHost, s.stream));
-  SEGTACO_HIP(hipStreamSynchronize(s.stream));
+  SEGTACO_RET(s.stager.d2h(C->vals, base + o_C, b_C, s.stream));
   return SEGTACO_OK;
 }
 

@@ -315,6 +315,22 @@ def test_primary_entry_points(name):
     assert ms > 0
 
 
+@pytest.mark.parametrize("N", [64, 37])
+def test_operator_surface_large_operands(N):
+    """Operands above the staging threshold (crd, vals, B and C of several
+    8 MiB chunks, sizes not a multiple of a chunk) go through the pinned
+    ring: the result matches the oracle on every call, and a second call that
+    reuses the ring gives the same bits."""
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 2, 262144, 262144)
+    B = S.gen_dense(12, 262144, N)
+    C1, _ = S.legacy("spmm_eb_sr", pos, crd, vals, B)
+    _check(pos, crd, vals, B, C1, what=f"staged N={N}")
+    C2, _ = S.legacy("spmm_rb_pr", pos, crd, vals, B)
+    _check(pos, crd, vals, B, C2, what=f"staged rb_pr N={N}")
+    C3, _ = S.legacy("spmm_eb_sr", pos, crd, vals, B)
+    assert np.array_equal(C1.view(np.uint32), C3.view(np.uint32))
+
+
 def test_all_operator_surface_names():
     pos, crd, vals, B = _case("n3_odd")
     pos2, crd2, vals2, B2 = _case("uniform_4096_n32")
