@@ -7,9 +7,10 @@
 // of 32. For a block whose entries span a column window of at most 256
 // (cmin .. cmax), the block is scattered into a dense 32 x W tile in LDS
 // (zeros elsewhere) and multiplied by rows cmin .. cmin+W of B with
-// v_mfma_f32_32x32x8{bf16_1k,f16}: B is staged in LDS 32 rows at a time as
-// it lies in memory (each lane gathers its four K-consecutive operands with
-// 2-byte reads), the next chunk's loads in flight during this chunk's MFMAs.
+// v_mfma_f32_32x32x8{bf16_1k,f16}: B is staged in LDS kBdKC rows at a time
+// as it lies in memory (each lane gathers its four K-consecutive operands
+// with 2-byte reads), the next chunk's loads in flight during this chunk's
+// MFMAs.
 // Each of the 4 waves owns CT tiles of 32 columns; accumulation is in
 // FP32 and every 16-bit product is exact in FP32. A block whose window is
 // wider falls back to a per-row walk on the vector ALUs (Neumaier
