@@ -574,16 +574,32 @@ int32_t segtaco_csr_validate_host(int32_t M, int32_t K, int32_t nnz,
     return set_error(SEGTACO_ERR_INVALID, "negative CSR size");
   if (!pos) return set_error(SEGTACO_ERR_INVALID, "null pos");
   if (pos[0] != 0) return set_error(SEGTACO_ERR_INVALID, "pos[0] = %d", pos[0]);
-  for (int32_t r = 0; r < M; ++r)
-    if (pos[r + 1] < pos[r])
-      return set_error(SEGTACO_ERR_INVALID, "pos decreases at row %d", r);
+  // Branch-free scans over blocks (they vectorise); the offending index is
+  // searched for only once a block fails.
+  constexpr int32_t kBlk = 4096;
+  for (int32_t r0 = 0; r0 < M; r0 += kBlk) {
+    const int32_t r1 = M - r0 < kBlk ? M : r0 + kBlk;
+    int bad = 0;
+    for (int32_t r = r0; r < r1; ++r) bad |= pos[r + 1] < pos[r];
+    if (bad)
+      for (int32_t r = r0; r < r1; ++r)
+        if (pos[r + 1] < pos[r])
+          return set_error(SEGTACO_ERR_INVALID, "pos decreases at row %d", r);
+  }
   if (pos[M] != nnz)
     return set_error(SEGTACO_ERR_INVALID, "pos[M] = %d != nnz = %d", pos[M], nnz);
   if (nnz > 0 && !crd) return set_error(SEGTACO_ERR_INVALID, "null crd");
-  for (int32_t p = 0; p < nnz; ++p)
-    if (crd[p] < 0 || crd[p] >= K)
-      return set_error(SEGTACO_ERR_INVALID, "crd[%d] = %d outside [0, %d)", p,
-                       crd[p], K);
+  const uint32_t k = (uint32_t)K;
+  for (int32_t p0 = 0; p0 < nnz; p0 += kBlk) {
+    const int32_t p1 = nnz - p0 < kBlk ? nnz : p0 + kBlk;
+    int bad = 0;
+    for (int32_t p = p0; p < p1; ++p) bad |= (uint32_t)crd[p] >= k;
+    if (bad)
+      for (int32_t p = p0; p < p1; ++p)
+        if (crd[p] < 0 || crd[p] >= K)
+          return set_error(SEGTACO_ERR_INVALID, "crd[%d] = %d outside [0, %d)", p,
+                           crd[p], K);
+  }
   return SEGTACO_OK;
 }
 

@@ -120,6 +120,23 @@ def test_validate_csr():
         S.validate_csr(3, 5, np.array([1, 2, 2, 5], np.int32), crd)
 
 
+def test_validate_csr_reports_the_first_offender_across_blocks():
+    """The scans run in blocks of 4096; the message names the exact index."""
+    M, K = 10000, 50
+    pos = np.arange(M + 1, dtype=np.int32)  # one entry per row
+    crd = (np.arange(M) % K).astype(np.int32)
+    S.validate_csr(M, K, pos, crd)
+    for bad_at, value in ((4096, K), (9999, -1), (5000, K + 7)):
+        c = crd.copy()
+        c[bad_at] = value
+        with pytest.raises(S.SegtacoError, match=rf"crd\[{bad_at}\] = {value} outside"):
+            S.validate_csr(M, K, pos, c)
+    p = pos.copy()
+    p[8193] = p[8192] - 1  # decreases at row 8192
+    with pytest.raises(S.SegtacoError, match="pos decreases at row 8192"):
+        S.validate_csr(M, K, p, crd)
+
+
 def test_legacy_rejects_malformed_operands_before_gpu_work():
     pos = np.array([0, 2, 3], np.int32)
     crd = np.array([0, 9, 1], np.int32)  # 9 >= K
