@@ -202,10 +202,11 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
         plan->col_lanes = 64 / plan->group;
       if (plan->col_lanes == 0) plan->col_lanes = qn < 32 ? qn : 32;
       if (plan->group == 0) plan->group = 64 / plan->col_lanes;
-      // tiles of 128 entries; at one column (SpMV, k_spmv) that is two
-      // consecutive entries per lane (power-law 262144^2: 32 us per call vs
-      // 34 / 41 / 49 us with 4 / 8 / 16 per lane)
-      if (plan->tile == 0) plan->tile = 128;
+      // tiles of 128 entries; at one column (SpMV, k_spmv) 256, four
+      // consecutive entries per lane read with one 128-bit load each for
+      // crd and vals (power-law 262144^2: 29.6 us per call vs 37.9 / 30.9 /
+      // 31.2 / 35.1 us with 1 / 2 / 8 / 16 per lane)
+      if (plan->tile == 0) plan->tile = N == 1 ? 256 : 128;
       break;
     case SEGTACO_EB_CS: {
       if (plan->col_lanes == 0) plan->col_lanes = kCsLanes;

@@ -154,17 +154,63 @@ def _spmv_case(name):
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("tile", [64, 192, 512, 1024])
+@pytest.mark.parametrize("tile", [64, 192, 256, 512, 1024])
 @pytest.mark.parametrize("case", ["powerlaw_20000_n1", "sparse_rows_n1", "long_rows_n1",
                                   "short_rows_n1"])
 def test_spmv_kernel_parity(case, tile):
     """eb_pr at N = 1 with tiles of 64 * E entries runs the SpMV kernel (E
-    consecutive entries per lane, one segmented scan over the wave)."""
+    consecutive entries per lane, one segmented scan over the wave; full
+    tiles read each lane's run with 64/128-bit loads)."""
     pos, crd, vals, B = _spmv_case(case)
     print(f"[spmv] {case} tile {tile} M={len(pos)-1} nnz={len(crd)}", flush=True)
     C, C2 = _device_run(pos, crd, vals, B, "eb_pr", dict(tile=tile), runs=2)
     _check(pos, crd, vals, B, C, what=f"spmv {case}/tile {tile}")
     assert np.array_equal(C, C2), "not deterministic"
+
+
+@pytest.mark.parametrize("has_vals", [True, False])
+@pytest.mark.parametrize("shift", [0, 1, 2])
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "fp16"])
+def test_spmv_kernel_alignment_and_storage(dtype, shift, has_vals):
+    """crd and vals starting `shift` elements past an aligned address: shift 0
+    takes the vector loads on full tiles, 1 and 2 the per-entry loads (16-bit
+    vals at shift 2 are 4-byte but not 8-byte aligned); every storage type,
+    with and without values, at the default tile and at 1024."""
+    t = torch()
+    pos, crd, vals, B = _spmv_case("powerlaw_20000_n1")
+    K, M, nnz = B.shape[0], len(pos) - 1, len(crd)
+    v = vals if has_vals else None
+    if dtype == "f32":
+        vr, Br = v, B
+        vt = None if v is None else t.from_numpy(v)
+        xt = t.from_numpy(B)
+    else:
+        vr = None if v is None else O.round_to(dtype, v)[1]
+        Br = O.round_to(dtype, B)[1]
+        vt = None if v is None else t.from_numpy(S.to_half(dtype, v).view(np.int16))
+        xt = t.from_numpy(S.to_half(dtype, B).view(np.int16))
+    dpos = dev(np.asarray(pos, np.int32))
+    dcrd = t.zeros(nnz + shift, dtype=t.int32, device="cuda")
+    dcrd[shift:] = t.from_numpy(np.asarray(crd, np.int32)).cuda()
+    dval = None
+    if vt is not None:
+        dval = t.zeros(nnz + shift, dtype=vt.dtype, device="cuda")
+        dval[shift:] = vt.cuda()
+    desc = S.csr_desc(M, K, nnz, dpos.data_ptr(), dcrd[shift:].data_ptr(),
+                      None if dval is None else dval[shift:].data_ptr(), S.DTYPES[dtype])
+    x = xt.cuda()
+    for tile in (0, 1024):
+        plan = S.make_plan("eb_pr", **({"tile": tile} if tile else {}))
+        print(f"[spmv] {dtype} shift {shift} vals {has_vals} tile {tile}", flush=True)
+        bd = S.DTYPES[dtype]
+        rp = S.plan_resolve(desc, 1, bd, plan, 1, 1)
+        wsb = S.workspace_bytes(desc, 1, rp)
+        ws = t.empty(max(wsb, 1), dtype=t.uint8, device="cuda")
+        y = t.full((M, 1), float("nan"), dtype=t.float32, device="cuda")
+        S.spmm(desc, x.data_ptr(), bd, 1, 1, y.data_ptr(), 1, plan, ws.data_ptr(), wsb,
+               stream_ptr())
+        t.cuda.synchronize()
+        _check(pos, crd, vr, Br, y.cpu().numpy(), what=f"spmv {dtype}/shift {shift}/tile {tile}")
 
 
 def test_spmv_kernel_strided_vectors():
