@@ -52,4 +52,14 @@ struct EbLayout {
 };
 EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile);
 
+// Compute units of the current device (256 on MI355X; queried once).
+int32_t device_cus();
+
+// Default element-balanced tile of a product with nnz entries: `tile`,
+// halved (not below 8) while the product would have fewer than two
+// workgroups of `tiles_per_block` tiles per compute unit. Each lane group
+// walks its tile serially, so on a small product a large tile leaves most of
+// the device idle and the call's time is one long serial walk.
+int32_t latency_tile(int64_t nnz, int32_t tile, int32_t tiles_per_block);
+
 }  // namespace segtaco

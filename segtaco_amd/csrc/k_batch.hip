@@ -126,8 +126,8 @@ hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid) {
 hipError_t launch_batch_main(int dt, const BatchDesc *d, int32_t n,
                              const int64_t *prefix, int64_t total,
                              unsigned long long *counter, int32_t grid,
-                             size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, counter);
+                             size_t lds, hipStream_t s, bool reset_counter) {
+  if (reset_counter) hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || total <= 0) return e;
   const dim3 g((unsigned)grid), b(kBlockThreads);

@@ -85,11 +85,13 @@ struct BatchDesc {
 
 // Persistent launch over `total` units (prefix[i] = first unit of product i,
 // non-decreasing, n entries); workgroups take units from *counter, which the
-// launcher resets first. grid = persistent workgroups, lds = dynamic LDS.
+// launcher resets first with a one-thread kernel unless reset_counter is
+// false (the caller has already put a zero there on the same stream).
+// grid = persistent workgroups, lds = dynamic LDS.
 hipError_t launch_batch_main(int dt, const BatchDesc *d, int32_t n,
                              const int64_t *prefix, int64_t total,
                              unsigned long long *counter, int32_t grid,
-                             size_t lds, hipStream_t s);
+                             size_t lds, hipStream_t s, bool reset_counter = true);
 hipError_t launch_batch_fixup(const BatchDesc *d, int32_t n,
                               const int64_t *fix_prefix, int64_t total,
                               hipStream_t s);

@@ -10,6 +10,8 @@
 // batched fix-up on that slot's stream. Admission stops at the first request
 // that does not fit (no overtaking), which keeps arrival order. Ticks on
 // different slots overlap on the device.
+#include <sys/prctl.h>
+
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -125,13 +127,28 @@ struct Req {
   double flops = 0.0;
 };
 
+// A tick's launch tables travel in one copy: one pinned host buffer and its
+// device twin, laid out as [work counter (zero)][unit prefix, n + 1]
+// [fix-up prefix, n + 1][descriptors, n] (PackLayout). Uploading the zero
+// counter with the tables replaces a reset kernel.
+struct PackLayout {
+  size_t prefix_off, fix_off, desc_off, total;
+  explicit PackLayout(size_t n) {
+    prefix_off = sizeof(unsigned long long);
+    fix_off = prefix_off + sizeof(int64_t) * (n + 1);
+    desc_off = align_up(fix_off + sizeof(int64_t) * (n + 1), 64);
+    total = desc_off + sizeof(BatchDesc) * n;
+  }
+};
+
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  char *d_pack = nullptr, *h_pack = nullptr;  // PackLayout(max_batch)
   BatchDesc *d_desc = nullptr, *h_desc = nullptr;
   int64_t *d_prefix = nullptr, *h_prefix = nullptr;
   int64_t *d_fix = nullptr, *h_fix = nullptr;
-  unsigned long long *d_counter = nullptr;
+  unsigned long long *d_counter = nullptr, *h_counter = nullptr;
   std::vector<int64_t> reqs;
   int64_t tick_id = -1;
   bool busy = false;
@@ -191,6 +208,9 @@ segtaco_plan_t plan_for(segtaco_server *s, MatrixRec &m, int32_t N) {
   if (m.nnz == 0 || segtaco_tune(&m.stats, N, s->cfg.dtype, &p) != SEGTACO_OK)
     p = segtaco_plan_t{SEGTACO_RB_SR, 0, 0, 0, 0, -1};
   p.vec = 0;  // widest the request's pointers allow
+  // A tick often carries a single product: size its tiles for latency.
+  if (p.schedule == SEGTACO_EB_SR && p.col_lanes > 0 && p.col_lanes <= 64 && p.tile > 0)
+    p.tile = latency_tile(m.nnz, p.tile, kWavesPerBlock * (64 / p.col_lanes));
   m.plan_by_n[N] = p;
   return p;
 }
@@ -325,7 +345,10 @@ void launch_tick(segtaco_server *s, int32_t si) {
   Slot &slot = s->slots[si];
   const int64_t tick = s->ticks + 1;
   std::vector<int64_t> chosen;
-  std::vector<const TileRows *> trs;
+  // Copies, not pointers into MatrixRec::tiles: a later request of this tick
+  // on the same matrix with another tile size appends to that vector, which
+  // may move its elements.
+  std::vector<TileRows> trs;
   bool issued = false;
   while (!s->pending.empty() && (int32_t)chosen.size() < s->cfg.max_batch) {
     const int64_t id = s->pending.front();
@@ -339,17 +362,20 @@ void launch_tick(segtaco_server *s, int32_t si) {
       continue;
     }
     m.inflight++;  // pin while its uploads are queued
-    TileRows *tr = nullptr;
-    if (!ensure_resident(s, m, p, si, tick, &tr, &issued)) {
-      m.inflight--;
-      break;
-    }
-    // Data written by a tick still running on another stream: wait for it.
+    // Data of this matrix written by a tick still running on another stream
+    // (its CSR upload or a tile-row array): this stream waits for it before
+    // it reads the matrix, including the tile-row pass ensure_resident may
+    // launch for a tile size the matrix does not have yet.
     if (m.ready_slot >= 0 && m.ready_slot != si) {
       const Slot &w = s->slots[m.ready_slot];
       if (w.busy && w.tick_id == m.ready_tick &&
           hipStreamWaitEvent(slot.stream, w.done, 0) != hipSuccess)
         s->fatal = SEGTACO_ERR_HIP;
+    }
+    TileRows *tr = nullptr;
+    if (!ensure_resident(s, m, p, si, tick, &tr, &issued)) {
+      m.inflight--;
+      break;
     }
     const size_t ws = carry_bytes(m.nnz, r.N, p);
     size_t off = 0;
@@ -363,7 +389,7 @@ void launch_tick(segtaco_server *s, int32_t si) {
     m.last_use = ++s->use_clock;
     s->pending.pop_front();
     chosen.push_back(id);
-    trs.push_back(tr);
+    trs.push_back(tr ? *tr : TileRows{});
   }
   if (chosen.empty() && !issued) return;
   size_t lds = 0;
@@ -379,8 +405,8 @@ void launch_tick(segtaco_server *s, int32_t si) {
     EbLayout lay{};
     fill_batch_desc(A, r.B, r.ldb, r.C, r.ldc, r.N, r.plan, &D, &units, &fix, &l, &lay);
     if (r.plan.schedule >= SEGTACO_EB_SR && m.nnz > 0) {
-      D.a.part = trs[i]->part;
-      D.a.tile_row = trs[i]->rows;
+      D.a.part = trs[i].part;
+      D.a.tile_row = trs[i].rows;
       const size_t half = r.ws_bytes / 2;
       D.a.carry_head = reinterpret_cast<float *>(s->inflight.base + r.ws_off);
       D.a.carry_tail = reinterpret_cast<float *>(s->inflight.base + r.ws_off + half);
@@ -406,18 +432,15 @@ void launch_tick(segtaco_server *s, int32_t si) {
     grid = 1024;
   }
   hipError_t e = hipSuccess;
+  // one copy: counter, both prefix arrays, the first n descriptors
+  *slot.h_counter = 0ull;
   if (n > 0)
-    e = hipMemcpyAsync(slot.d_desc, slot.h_desc, sizeof(BatchDesc) * n,
-                       hipMemcpyHostToDevice, slot.stream);
-  if (e == hipSuccess && n > 0)
-    e = hipMemcpyAsync(slot.d_prefix, slot.h_prefix, sizeof(int64_t) * (n + 1),
-                       hipMemcpyHostToDevice, slot.stream);
-  if (e == hipSuccess && n > 0)
-    e = hipMemcpyAsync(slot.d_fix, slot.h_fix, sizeof(int64_t) * (n + 1),
+    e = hipMemcpyAsync(slot.d_pack, slot.h_pack,
+                       reinterpret_cast<char *>(slot.h_desc + n) - slot.h_pack,
                        hipMemcpyHostToDevice, slot.stream);
   if (e == hipSuccess && n > 0)
     e = launch_batch_main(s->cfg.dtype, slot.d_desc, n, slot.d_prefix, total,
-                          slot.d_counter, grid, lds, slot.stream);
+                          slot.d_counter, grid, lds, slot.stream, /*reset_counter=*/false);
   if (e == hipSuccess && n > 0)
     e = launch_batch_fixup(slot.d_desc, n, slot.d_fix, total_fix, slot.stream);
   if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);
@@ -434,6 +457,10 @@ void launch_tick(segtaco_server *s, int32_t si) {
 
 void scheduler(segtaco_server *s) {
   (void)hipSetDevice(s->device);
+  // The waits below are a few microseconds long while a tick runs; Linux
+  // otherwise rounds such a timed wait up by the default 50 us timer slack,
+  // which lands directly on every request's latency.
+  (void)prctl(PR_SET_TIMERSLACK, 1UL, 0UL, 0UL, 0UL);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     bool progressed = false;
@@ -469,8 +496,10 @@ void scheduler(segtaco_server *s) {
       s->cv_done.notify_all();
       continue;
     }
+    // A running tick's completion event is polled every few microseconds;
+    // an idle scheduler sleeps until a submission (or stop) wakes it.
     if (!progressed)
-      s->cv_work.wait_for(lk, std::chrono::microseconds(any_busy ? 20 : 500));
+      s->cv_work.wait_for(lk, std::chrono::microseconds(any_busy ? 2 : 500));
   }
 }
 
@@ -495,13 +524,18 @@ int32_t segtaco_server_create(const segtaco_serve_config_t *cfg,
   for (auto &sl : s->slots) {
     SEGTACO_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
     SEGTACO_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-    SEGTACO_HIP(hipMalloc(&sl.d_desc, sizeof(BatchDesc) * n));
-    SEGTACO_HIP(hipMalloc(&sl.d_prefix, sizeof(int64_t) * (n + 1)));
-    SEGTACO_HIP(hipMalloc(&sl.d_fix, sizeof(int64_t) * (n + 1)));
-    SEGTACO_HIP(hipMalloc(&sl.d_counter, sizeof(unsigned long long)));
-    SEGTACO_HIP(hipHostMalloc(&sl.h_desc, sizeof(BatchDesc) * n, hipHostMallocDefault));
-    SEGTACO_HIP(hipHostMalloc(&sl.h_prefix, sizeof(int64_t) * (n + 1), hipHostMallocDefault));
-    SEGTACO_HIP(hipHostMalloc(&sl.h_fix, sizeof(int64_t) * (n + 1), hipHostMallocDefault));
+    const PackLayout L(n);
+    SEGTACO_HIP(hipMalloc(&sl.d_pack, L.total));
+    SEGTACO_HIP(hipHostMalloc(&sl.h_pack, L.total, hipHostMallocDefault));
+    std::memset(sl.h_pack, 0, L.total);
+    sl.d_counter = reinterpret_cast<unsigned long long *>(sl.d_pack);
+    sl.h_counter = reinterpret_cast<unsigned long long *>(sl.h_pack);
+    sl.d_prefix = reinterpret_cast<int64_t *>(sl.d_pack + L.prefix_off);
+    sl.h_prefix = reinterpret_cast<int64_t *>(sl.h_pack + L.prefix_off);
+    sl.d_fix = reinterpret_cast<int64_t *>(sl.d_pack + L.fix_off);
+    sl.h_fix = reinterpret_cast<int64_t *>(sl.h_pack + L.fix_off);
+    sl.d_desc = reinterpret_cast<BatchDesc *>(sl.d_pack + L.desc_off);
+    sl.h_desc = reinterpret_cast<BatchDesc *>(sl.h_pack + L.desc_off);
   }
   segtaco_server *raw = s.release();
   raw->th = std::thread(scheduler, raw);
@@ -716,13 +750,8 @@ int32_t segtaco_server_destroy(segtaco_server_t s) {
     (void)hipStreamSynchronize(sl.stream);
     (void)hipStreamDestroy(sl.stream);
     (void)hipEventDestroy(sl.done);
-    (void)hipFree(sl.d_desc);
-    (void)hipFree(sl.d_prefix);
-    (void)hipFree(sl.d_fix);
-    (void)hipFree(sl.d_counter);
-    (void)hipHostFree(sl.h_desc);
-    (void)hipHostFree(sl.h_prefix);
-    (void)hipHostFree(sl.h_fix);
+    (void)hipFree(sl.d_pack);
+    (void)hipHostFree(sl.h_pack);
   }
   for (auto &m : s->mats) {
     (void)hipHostFree(m->h_pos);

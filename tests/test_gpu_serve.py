@@ -170,3 +170,32 @@ def test_poisson_replay_latency_stats():
                            O.spmm_absbound(pos, crd, vals, B))
         assert err <= 1e-5
     server.destroy()
+
+
+def test_one_matrix_many_widths_across_streams():
+    """One matrix served at column counts whose plans use different tile sizes,
+    one request per tick on two streams: the second stream builds its tile
+    rows while the first stream's upload tick may still run, and one tick may
+    add several tile sizes to the matrix. Every result matches the oracle and
+    the same product run alone."""
+    rng = np.random.default_rng(23)
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 29, 6000, 6000, mean=40)
+    mats = [(pos, crd, vals, 6000)]
+    Ns = [16, 64, 128, 32, 16, 128, 64, 32] * 3
+    for max_batch in (1, 8):
+        server = S.Server(cache_bytes=256 * MiB, inflight_bytes=256 * MiB,
+                          max_batch=max_batch, streams=2)
+        server.add_matrix(pos, crd, vals, 6000)
+        tiles = {server.plan(0, N).tile for N in set(Ns)}
+        print(f"[serve widths] max_batch {max_batch} tiles {sorted(tiles)}", flush=True)
+        assert len(tiles) > 1, "widths should map to different tile sizes"
+        reqs = [(0, N, rng.uniform(-1, 1, (6000, N)).astype(np.float32)) for N in Ns]
+        dev = [(dense_dev(B), torch().full((6000, N), float("nan"), dtype=torch().float32,
+                                           device="cuda")) for _, N, B in reqs]
+        torch().cuda.synchronize()
+        for (mi, N, _), (Bd, C) in zip(reqs, dev):
+            server.submit(mi, Bd.data_ptr(), N, N, C.data_ptr(), N)
+        server.drain(120.0)
+        assert server.stats()["completed"] == len(reqs)
+        _check_results(server, mats, reqs, dev, "f32")
+        server.destroy()
