@@ -130,11 +130,15 @@ struct Req {
 // A tick's launch tables travel in one copy: one pinned host buffer and its
 // device twin, laid out as [work counter (zero)][unit prefix, n + 1]
 // [fix-up prefix, n + 1][descriptors, n] (PackLayout). Uploading the zero
-// counter with the tables replaces a reset kernel.
+// counter with the tables replaces a reset kernel. The counter has a
+// 256-byte block to itself: every workgroup increments it atomically and
+// then bisects the prefix array, and with both in one cache line each
+// bisection read contended with the atomics (a power-law 262144^2 BF16
+// N = 128 burst ran at 5150 instead of 6570 GFLOP/s).
 struct PackLayout {
   size_t prefix_off, fix_off, desc_off, total;
   explicit PackLayout(size_t n) {
-    prefix_off = sizeof(unsigned long long);
+    prefix_off = 256;
     fix_off = prefix_off + sizeof(int64_t) * (n + 1);
     desc_off = align_up(fix_off + sizeof(int64_t) * (n + 1), 64);
     total = desc_off + sizeof(BatchDesc) * n;
@@ -455,6 +459,12 @@ void launch_tick(segtaco_server *s, int32_t si) {
   s->max_conc = std::max(s->max_conc, busy);
 }
 
+// Interval at which the scheduler polls a running tick's completion event.
+#ifndef SEGTACO_POLL_US
+#define SEGTACO_POLL_US 2
+#endif
+constexpr int kPollUs = SEGTACO_POLL_US;
+
 void scheduler(segtaco_server *s) {
   (void)hipSetDevice(s->device);
   // The waits below are a few microseconds long while a tick runs; Linux
@@ -499,7 +509,7 @@ void scheduler(segtaco_server *s) {
     // A running tick's completion event is polled every few microseconds;
     // an idle scheduler sleeps until a submission (or stop) wakes it.
     if (!progressed)
-      s->cv_work.wait_for(lk, std::chrono::microseconds(any_busy ? 2 : 500));
+      s->cv_work.wait_for(lk, std::chrono::microseconds(any_busy ? kPollUs : 500));
   }
 }
 
