@@ -522,7 +522,15 @@ def main():
     plans = (parse_plans(args.plans) if args.plans
              else [(n, S.make_plan(n)) for n in args.schedules.split(",")])
     head, cbytes = run_leg(torch, pl, N, "f32", plans, args.steps, args.warmup, stream, 3)
-    best = min(head, key=lambda k: head[k]["ms"])
+    fastest = min(head, key=lambda k: head[k]["ms"])
+    # The headline is the schedule the autotuner picks for this matrix (the
+    # plan a caller gets), unless the sweep found one more than 2 % faster:
+    # eb_sr and eb_pr sit within 1 % of each other here and swap places from
+    # run to run.
+    inv = {v: k for k, v in S.SCHEDULES.items()}
+    tuned = inv.get(S.tune_model(S.row_stats(pl.pos, pl.K), N).schedule)
+    best = (tuned if tuned in head and head[tuned]["ms"] <= 1.02 * head[fastest]["ms"]
+            else fastest)
     r = head[best]
     ms = r["ms"]
     flops = 2.0 * pl.nnz * N
@@ -544,7 +552,9 @@ def main():
         "data": "synthetic (seeded splitmix64 generator)",
         "config": {"workload": "powerlaw_262144_n128_f32", "M": pl.M, "K": pl.K, "N": N,
                    "nnz": pl.nnz, "row_lengths": "log-normal mean 16 sigma 1.5 cap 8192",
-                   "schedule": best, "plan": r["plan"]},
+                   "schedule": best, "plan": r["plan"],
+                   "sweep_fastest": {"schedule": fastest,
+                                     "ms_per_step": round(head[fastest]["ms"], 5)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -536,11 +536,15 @@ int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,
     return set_error(SEGTACO_ERR_INVALID, "segtaco_spmm_stage_times: bad argument");
   hipStream_t stream = static_cast<hipStream_t>(stream_v);
   // All reps are enqueued back to back (as in a steady loop of calls), each
-  // with its own four events; the host waits once at the end.
+  // with its own four events; the host waits once at the end. The events
+  // between kernels skip the system-scope fence (an L2 writeback and
+  // invalidate that a loop of untimed calls does not have); the last one
+  // keeps it, so waiting on it covers all the work.
   std::vector<hipEvent_t> ev((size_t)reps * 4, nullptr);
   int32_t st = SEGTACO_OK;
   for (auto &e : ev) {
-    const hipError_t he = hipEventCreate(&e);
+    const unsigned flags = &e == &ev.back() ? hipEventDefault : hipEventDisableSystemFence;
+    const hipError_t he = hipEventCreateWithFlags(&e, flags);
     if (he != hipSuccess) {
       e = nullptr;
       st = hip_error(he, "hipEventCreate");
