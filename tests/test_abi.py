@@ -195,6 +195,35 @@ def test_eb_cs_plan_resolution():
     assert p.tile == 32
 
 
+def test_eb_sr_default_tile_follows_product_size():
+    """eb_sr's default tile is 128 entries, halved (down to 8) while the
+    product would fill fewer than two 4-wave workgroups per compute unit (256
+    on MI355X, also the answer without a device): a lane group walks its tile
+    serially, so a small product needs short tiles to spread over the device.
+    Explicit tiles are kept."""
+    def tile(nnz, N):
+        return S.plan_resolve(_desc(M=max(1, nnz // 16), K=1000, nnz=nnz), N, S.DT_F32,
+                              S.make_plan("eb_sr")).tile
+
+    # N = 128 FP32: 32 lanes per entry, 2 lane groups per wave, 8 tiles per workgroup
+    want = 2 * 256 * 8
+    assert tile(4204209, 128) == 128           # the power-law headline keeps 128
+    assert tile(128 * want, 128) == 128        # exactly enough tiles at 128
+    assert tile(128 * (want - 1), 128) == 64  # one tile short at 128
+    assert tile(15591, 128) == 8               # floor
+    # N = 16 FP32: 4 lanes per entry, 16 groups per wave, 64 tiles per workgroup
+    assert tile(100000, 16) == 8
+    assert tile(64 * 2 * 256 * 64, 16) == 64
+    # the smallest tile T with ceil(nnz / T) >= want, or 8
+    for nnz in (1, 1000, 50000, 300000, 2000000):
+        t = tile(nnz, 128)
+        assert t in (8, 16, 32, 64, 128)
+        assert t == 128 or -(-nnz // t) >= want or t == 8
+        assert t == 8 or -(-nnz // (t // 2)) >= want or t == 128
+    assert S.plan_resolve(_desc(nnz=15591), 128, S.DT_F32,
+                          S.make_plan("eb_sr", tile=512)).tile == 512
+
+
 def test_bd_plan_resolution():
     """bd: 16-bit storage only, 32-row blocks, lanes and groups fixed at 1."""
     B16 = _desc(M=1000, K=1000, nnz=20000, dtype=S.DT_BF16)
