@@ -93,7 +93,7 @@ void fill_batch_desc(const segtaco_csr_t &A, const void *B, int64_t ldb,
   a.vals = A.vals;
   a.B = B;
   a.ldb = ldb;
-  // judged for 4-byte elements, which covers the 16-bit storage types too
+  // checked for 4-byte elements, which covers the 16-bit storage types too
   a.b_off32 = b_off32_ok(A.K, ldb, 4);
   a.C = C;
   a.ldc = ldc;
