@@ -212,7 +212,10 @@ typedef struct segtaco_plan {
                              1..64; the column tile is col_lanes * vec */
   int32_t group;          /* *_pr: nnz slots reduced together (power of two);
                              rb_sr: rows per lane group; 0 = auto */
-  int32_t tile;           /* eb_*: nnz per tile (multiple of 8); 0 = auto */
+  int32_t tile;           /* eb_*: nnz per tile (multiple of 8); 0 = auto:
+                             eb_sr 128, halved (down to 8) while the product
+                             would fill fewer than two workgroups per compute
+                             unit; eb_pr 128 (256 at N = 1) */
   int32_t comp_threshold; /* rows with more nnz than this use compensated
                              (Neumaier) summation; 0 = never; -1 = default
                              (1024 for 16-bit storage, never for FP32) */
