@@ -152,6 +152,9 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
                 int64_t ldb, int64_t ldc, const void *B, const void *C,
                 segtaco_plan_t *plan) {
   if (!A || !plan) return set_error(SEGTACO_ERR_INVALID, "null argument");
+  if (A->nnz > kMaxNnz)
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "nnz %d above the supported %d", A->nnz,
+                     kMaxNnz);
   if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_BD)
     return set_error(SEGTACO_ERR_INVALID, "unknown schedule %d",
                      plan->schedule);
@@ -281,6 +284,9 @@ int32_t check_csr(const segtaco_csr_t *A) {
   if (!A) return set_error(SEGTACO_ERR_INVALID, "null CSR");
   if (A->M < 0 || A->K < 0 || A->nnz < 0)
     return set_error(SEGTACO_ERR_INVALID, "negative CSR size");
+  if (A->nnz > kMaxNnz)
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "nnz %d above the supported %d", A->nnz,
+                     kMaxNnz);
   if (A->M > 0 && !A->pos) return set_error(SEGTACO_ERR_INVALID, "null pos");
   if (A->nnz > 0 && !A->crd) return set_error(SEGTACO_ERR_INVALID, "null crd");
   return SEGTACO_OK;

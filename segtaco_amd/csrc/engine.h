@@ -52,6 +52,11 @@ struct EbLayout {
 };
 EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile);
 
+// Largest nnz the engine accepts. Entry positions are 32-bit, and kernels
+// form positions up to one tile (at most 2^22 entries) plus a staging chunk
+// past a tile's start; below this bound none of them can wrap.
+constexpr int32_t kMaxNnz = 0x7fffffff - (1 << 23);
+
 // Compute units of the current device (256 on MI355X; queried once).
 int32_t device_cus();
 

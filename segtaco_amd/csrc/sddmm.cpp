@@ -35,6 +35,9 @@ int32_t segtaco_sddmm(const segtaco_csr_t *S, const void *X, int64_t ldx,
                       void *stream_v) {
   if (!S || S->M < 0 || S->K < 0 || S->nnz < 0 || J < 0)
     return set_error(SEGTACO_ERR_INVALID, "segtaco_sddmm: bad operand shape");
+  if (S->nnz > kMaxNnz)
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "segtaco_sddmm: nnz %d above the supported %d",
+                     S->nnz, kMaxNnz);
   if (dtype < SEGTACO_DT_F32 || dtype > SEGTACO_DT_F16)
     return set_error(SEGTACO_ERR_INVALID, "segtaco_sddmm: unknown dtype %d", dtype);
   if (S->vals && S->vals_dtype != dtype)

@@ -558,6 +558,8 @@ int32_t segtaco_server_add_matrix(segtaco_server_t s, int32_t M, int32_t K,
                                   const int32_t *crd, const void *vals,
                                   int32_t *matrix_id) {
   if (!s || !matrix_id) return set_error(SEGTACO_ERR_INVALID, "null argument");
+  if (nnz > kMaxNnz)
+    return set_error(SEGTACO_ERR_UNSUPPORTED, "nnz %d above the supported %d", nnz, kMaxNnz);
   SEGTACO_RET(segtaco_csr_validate_host(M, K, nnz, pos, crd));
   auto m = std::make_unique<MatrixRec>();
   m->M = M;

@@ -195,6 +195,21 @@ def test_eb_cs_plan_resolution():
     assert p.tile == 32
 
 
+def test_nnz_bound_leaves_room_for_32bit_tile_positions():
+    """Entry positions are 32-bit and kernels form positions up to one tile
+    (at most 2^22) plus a staging chunk past a tile's start: the engine takes
+    nnz up to 2^31 - 1 - 2^23 and refuses more with UNSUPPORTED (4)."""
+    limit = 0x7FFFFFFF - (1 << 23)
+    for sched in ("rb_sr", "eb_sr", "eb_pr"):
+        p = S.plan_resolve(_desc(M=1 << 26, K=1 << 20, nnz=limit), 32, S.DT_F32,
+                           S.make_plan(sched, **({"tile": 1 << 22} if sched != "rb_sr" else {})))
+        assert p.schedule == S.SCHEDULES[sched]
+        with pytest.raises(S.SegtacoError) as e:
+            S.plan_resolve(_desc(M=1 << 26, K=1 << 20, nnz=limit + 1), 32, S.DT_F32,
+                           S.make_plan(sched))
+        assert e.value.status == 4 and "nnz" in str(e.value)
+
+
 def test_eb_sr_default_tile_follows_product_size():
     """eb_sr's default tile is 128 entries, halved (down to 8) while the
     product would fill fewer than two 4-wave workgroups per compute unit (256
