@@ -1,0 +1,90 @@
+"""Seeded random sweep through the C-ABI: random CSR structures (empty rows,
+short rows, log-normal rows, sparse long rows, K down to 1), every schedule,
+FP32 / BF16 / FP16 storage, column counts from 1 to 300, padded leading
+dimensions, with and without values, and tiles / groups drawn from the
+values plan resolution accepts (a combination it refuses is skipped).
+
+Tolerances as everywhere: FP32 storage within 1e-5 of the FP64 oracle on the
+oracle's scaled metric; 16-bit storage within 1e-5 of FP64 on the rounded
+inputs. Each case prints its seed and plan first, so a failure names it.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+import segtaco_amd as S
+from gpu_util import DeviceCSR, dense_dev, run_spmm
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = list(range(48))
+
+
+def _structure(rng):
+    M = int(rng.integers(1, 1500))
+    K = int(rng.integers(1, 1500))
+    kind = int(rng.integers(0, 4))
+    if kind == 0:
+        lens = rng.integers(0, 8, M)
+    elif kind == 1:
+        lens = rng.poisson(20, M)
+    elif kind == 2:
+        lens = rng.lognormal(2.0, 1.5, M).astype(np.int64)
+    else:
+        lens = np.where(rng.random(M) < 0.9, 0, rng.integers(1, 1400, M))
+    lens = np.minimum(lens, K)
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    crd = np.concatenate([np.sort(rng.choice(K, int(l), replace=False)) for l in lens]
+                         + [np.zeros(0, np.int64)]).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(crd)).astype(np.float32)
+    return pos, crd, vals, K
+
+
+def _plan(rng, dtype):
+    scheds = ["rb_sr", "rb_pr", "eb_sr", "eb_pr", "eb_cs"] + (["bd"] if dtype != "f32" else [])
+    name = str(rng.choice(scheds))
+    kw = {}
+    if name == "rb_sr":
+        kw["group"] = int(rng.choice([0, 1, 2, 4, 16]))
+    elif name == "rb_pr":
+        kw["group"] = int(rng.choice([0, 2, 8, 32]))
+    elif name == "eb_sr":
+        kw["tile"] = int(rng.choice([0, 8, 16, 40, 128, 200, 512]))
+    elif name == "eb_pr":
+        kw["tile"] = int(rng.choice([0, 64, 128, 256, 1024, 96]))
+    elif name == "eb_cs":
+        kw["tile"] = int(rng.choice([0, 32, 64, 256]))
+    return name, {k: v for k, v in kw.items() if v}
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_structures_plans_and_shapes(seed):
+    rng = np.random.default_rng(zlib.crc32(f"fuzz{seed}".encode()))
+    pos, crd, vals, K = _structure(rng)
+    dtype = str(rng.choice(["f32", "bf16", "fp16"]))
+    N = int(rng.choice([1, 2, 3, 7, 16, 31, 64, 100, 128, 256, 300]))
+    ldb = N + int(rng.choice([0, 0, 1, 3, 8]))
+    ldc = N + int(rng.choice([0, 0, 2, 5, 8]))
+    has_vals = bool(rng.random() < 0.85)
+    name, kw = _plan(rng, dtype)
+    M = len(pos) - 1
+    print(f"[fuzz] seed {seed}: M={M} K={K} nnz={len(crd)} {dtype} N={N} ldb={ldb} ldc={ldc} "
+          f"vals={has_vals} {name} {kw}", flush=True)
+    v = vals if has_vals else None
+    A = DeviceCSR(pos, crd, v, K, dtype=dtype)
+    plan = S.make_plan(name, **kw)
+    try:
+        S.plan_resolve(A.desc, N, S.DTYPES[dtype], plan, ldb, ldc)
+    except S.SegtacoError as e:
+        pytest.skip(f"plan refused: {e}")
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    Bp = np.zeros((K, ldb), np.float32)
+    Bp[:, :N] = B
+    C = run_spmm(A, dense_dev(Bp, dtype), N, plan, ldb=ldb, ldc=ldc, b_dtype=dtype)
+    Cg = C[:M, :N].cpu().numpy()
+    vr = v if (dtype == "f32" or v is None) else O.round_to(dtype, v)[1]
+    Br = B if dtype == "f32" else O.round_to(dtype, B)[1]
+    err = O.scaled_err(Cg, O.spmm_f64(pos, crd, vr, Br), O.spmm_absbound(pos, crd, vr, Br))
+    assert err <= 1e-5, f"seed {seed}: scaled error {err:.3e}"
