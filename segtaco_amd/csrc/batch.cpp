@@ -13,6 +13,7 @@
 
 #include "engine.h"
 #include "launch.h"
+#include "spmv.h"
 #include "segtaco/autotune.h"
 #include "segtaco/batch.h"
 
@@ -36,6 +37,7 @@ struct segtaco_batch {
   size_t ws_bytes = 0;
   size_t lds = 0;
   int32_t grid = 0;
+  bool sv = false;  // some product runs the N = 1 form of eb_pr
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -124,7 +126,9 @@ void fill_batch_desc(const segtaco_csr_t &A, const void *B, int64_t ldb,
       a.nblocks = plan.schedule == SEGTACO_EB_SR
                       ? (int32_t)ceil_div(lay->ntiles, tiles_per_group_block)
                       : (int32_t)ceil_div(lay->ntiles, kWavesPerBlock);
-      *lds = plan.schedule == SEGTACO_EB_SR ? eb_sr_lds_bytes(a) : eb_pr_lds_bytes(a);
+      *lds = plan.schedule == SEGTACO_EB_SR ? eb_sr_lds_bytes(a)
+             : spmv_form(a, plan.vec)        ? kSvLdsBytes
+                                             : eb_pr_lds_bytes(a);
       *fix_units = ceil_div(lay->ntiles, tiles_per_group_block) * a.panels;
       break;
     }
@@ -210,6 +214,7 @@ int32_t segtaco_batch_create(const segtaco_batch_item_t *items, int32_t n,
     fill_batch_desc(A, it.B, it.ldb, it.C, it.ldc, it.N, plan, &D, &units,
                     &fix_units, &lds, &lay[i]);
     if (lds > b->lds) b->lds = lds;
+    b->sv = b->sv || batch_uses_spmv(D);
     if (plan.schedule >= SEGTACO_EB_SR) {
       ws_off[i] = ws_total;
       ws_total += align_up(lay[i].total, 256);
@@ -266,7 +271,7 @@ int32_t segtaco_batch_create(const segtaco_batch_item_t *items, int32_t n,
   if ((e = hipMemcpy(b->d_fix_prefix, b->fix_prefix.data(), sizeof(int64_t) * (n + 1),
                      hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e, "hipMemcpy(fix prefix)");
-  if ((e = batch_main_grid(b_dtype, b->lds, &b->grid)) != hipSuccess)
+  if ((e = batch_main_grid(b_dtype, b->lds, &b->grid, b->sv)) != hipSuccess)
     return fail(e, "batch_main_grid");
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return fail(e, "hipStreamSynchronize");
   *out = b;
@@ -278,7 +283,8 @@ int32_t segtaco_batch_run(segtaco_batch_t b, void *stream_v) {
   hipStream_t s = static_cast<hipStream_t>(stream_v);
   if (b->n == 0) return SEGTACO_OK;
   SEGTACO_HIP(launch_batch_main(b->dt, b->d_desc, b->n, b->d_prefix, b->total_units,
-                                b->d_counter, b->grid, b->lds, s));
+                                b->d_counter, b->grid, b->lds, s, /*reset_counter=*/true,
+                                b->sv));
   SEGTACO_HIP(launch_batch_fixup(b->d_desc, b->n, b->d_fix_prefix, b->total_fix, s));
   return SEGTACO_OK;
 }

@@ -13,10 +13,14 @@
 #include "sched_eb.h"
 #include "sched_rb.h"
 #include "segtaco/spmm.h"
+#include "spmv_body.h"
 
 namespace segtaco {
 
-template <int DT, int V>
+// kSv: the batch holds a product in the N = 1 form of eb_pr (spmv_form);
+// only then is that body compiled in, since it costs the 16-bit kernel
+// registers and 3 % of served throughput.
+template <int DT, int V, bool kSv>
 __device__ __forceinline__ void run_body(int32_t sched, const KArgs &a,
                                          int32_t blk, int32_t panel,
                                          int32_t *smem) {
@@ -24,22 +28,34 @@ __device__ __forceinline__ void run_body(int32_t sched, const KArgs &a,
     case SEGTACO_RB_SR: rb_sr_body<DT, V>(a, blk, panel); break;
     case SEGTACO_RB_PR: rb_pr_body<DT, V>(a, blk, panel); break;
     case SEGTACO_EB_SR: eb_sr_body<DT, V>(a, blk, panel, smem); break;
-    default: eb_pr_body<DT, V>(a, blk, panel, smem); break;
+    default:
+      if constexpr (kSv && V == 1) {
+        // eb_pr at N = 1: the form a standalone call takes (one wave per
+        // tile, as in k_spmv), so the product's bits match a single call
+        if (spmv_form(a, 1)) {
+          const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+          const int32_t t = blk * kWavesPerBlock + wave;
+          if (t < a.ntiles) spmv_tile_rt<DT>(a, t, smem + wave * kSvWindow);
+          break;
+        }
+      }
+      eb_pr_body<DT, V>(a, blk, panel, smem);
+      break;
   }
 }
 
-template <int DT>
+template <int DT, bool kSv>
 __device__ __forceinline__ void dispatch(int32_t sched, int32_t vec,
                                          const KArgs &a, int32_t blk,
                                          int32_t panel, int32_t *smem) {
   if (vec == 4) {
-    run_body<DT, 4>(sched, a, blk, panel, smem);
+    run_body<DT, 4, kSv>(sched, a, blk, panel, smem);
   } else if (vec == 2) {
-    run_body<DT, 2>(sched, a, blk, panel, smem);
+    run_body<DT, 2, kSv>(sched, a, blk, panel, smem);
   } else if (vec == 1) {
-    run_body<DT, 1>(sched, a, blk, panel, smem);
+    run_body<DT, 1, kSv>(sched, a, blk, panel, smem);
   } else if constexpr (DT != 0) {
-    if (vec == 8) run_body<DT, 8>(sched, a, blk, panel, smem);
+    if (vec == 8) run_body<DT, 8, kSv>(sched, a, blk, panel, smem);
   }
 }
 
@@ -59,8 +75,11 @@ __device__ __forceinline__ int32_t find_product(const int64_t *prefix,
 
 __global__ void k_batch_reset(unsigned long long *counter) { *counter = 0ull; }
 
-template <int DT>
-__global__ __launch_bounds__(kBlockThreads) void k_batch_main(
+// FP32: held at 4 waves per SIMD (128 VGPRs), which every FP32 body fits;
+// the 16-bit forms need more registers and run at 3.
+template <int DT, bool kSv>
+__global__ __launch_bounds__(kBlockThreads)
+__attribute__((amdgpu_waves_per_eu(DT == 0 ? 4 : 1))) void k_batch_main(
     const BatchDesc *__restrict__ d, int32_t n,
     const int64_t *__restrict__ prefix, int64_t total,
     unsigned long long *counter) {
@@ -83,7 +102,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_main(
     const int64_t local = u - D.unit_begin;
     const int32_t panel = (int32_t)(local % a.panels);
     const int32_t blk = (int32_t)(local / a.panels);
-    dispatch<DT>(D.schedule, D.vec, a, blk, panel, smem);
+    dispatch<DT, kSv>(D.schedule, D.vec, a, blk, panel, smem);
   }
 }
 
@@ -106,15 +125,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_batch_fixup(
   }
 }
 
-hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid) {
+hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid, bool sv) {
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
-  const void *fn = dt == 0   ? (const void *)k_batch_main<0>
-                   : dt == 1 ? (const void *)k_batch_main<1>
-                             : (const void *)k_batch_main<2>;
+  const void *fn =
+      sv ? (dt == 0 ? (const void *)k_batch_main<0, true>
+            : dt == 1 ? (const void *)k_batch_main<1, true>
+                      : (const void *)k_batch_main<2, true>)
+         : (dt == 0 ? (const void *)k_batch_main<0, false>
+            : dt == 1 ? (const void *)k_batch_main<1, false>
+                      : (const void *)k_batch_main<2, false>);
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlockThreads,
                                                    lds);
   if (e != hipSuccess) return e;
@@ -126,15 +149,18 @@ hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid) {
 hipError_t launch_batch_main(int dt, const BatchDesc *d, int32_t n,
                              const int64_t *prefix, int64_t total,
                              unsigned long long *counter, int32_t grid,
-                             size_t lds, hipStream_t s, bool reset_counter) {
+                             size_t lds, hipStream_t s, bool reset_counter, bool sv) {
   if (reset_counter) hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || total <= 0) return e;
   const dim3 g((unsigned)grid), b(kBlockThreads);
-  switch (dt) {
-    case 0: hipLaunchKernelGGL(k_batch_main<0>, g, b, lds, s, d, n, prefix, total, counter); break;
-    case 1: hipLaunchKernelGGL(k_batch_main<1>, g, b, lds, s, d, n, prefix, total, counter); break;
-    case 2: hipLaunchKernelGGL(k_batch_main<2>, g, b, lds, s, d, n, prefix, total, counter); break;
+  switch (dt * 2 + (sv ? 1 : 0)) {
+    case 0: hipLaunchKernelGGL((k_batch_main<0, false>), g, b, lds, s, d, n, prefix, total, counter); break;
+    case 1: hipLaunchKernelGGL((k_batch_main<0, true>), g, b, lds, s, d, n, prefix, total, counter); break;
+    case 2: hipLaunchKernelGGL((k_batch_main<1, false>), g, b, lds, s, d, n, prefix, total, counter); break;
+    case 3: hipLaunchKernelGGL((k_batch_main<1, true>), g, b, lds, s, d, n, prefix, total, counter); break;
+    case 4: hipLaunchKernelGGL((k_batch_main<2, false>), g, b, lds, s, d, n, prefix, total, counter); break;
+    case 5: hipLaunchKernelGGL((k_batch_main<2, true>), g, b, lds, s, d, n, prefix, total, counter); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -87,11 +87,14 @@ struct BatchDesc {
 // non-decreasing, n entries); workgroups take units from *counter, which the
 // launcher resets first with a one-thread kernel unless reset_counter is
 // false (the caller has already put a zero there on the same stream).
-// grid = persistent workgroups, lds = dynamic LDS.
+// grid = persistent workgroups (batch_main_grid), lds = dynamic LDS; sv: some
+// product takes the N = 1 form of eb_pr (batch_uses_spmv), which selects the
+// kernel that carries that body.
 hipError_t launch_batch_main(int dt, const BatchDesc *d, int32_t n,
                              const int64_t *prefix, int64_t total,
                              unsigned long long *counter, int32_t grid,
-                             size_t lds, hipStream_t s, bool reset_counter = true);
+                             size_t lds, hipStream_t s, bool reset_counter = true,
+                             bool sv = false);
 hipError_t launch_batch_fixup(const BatchDesc *d, int32_t n,
                               const int64_t *fix_prefix, int64_t total,
                               hipStream_t s);
@@ -104,6 +107,6 @@ void fill_batch_desc(const segtaco_csr_t &A, const void *B, int64_t ldb,
                      const segtaco_plan_t &plan, BatchDesc *D, int64_t *units,
                      int64_t *fix_units, size_t *lds, EbLayout *lay);
 // Workgroups of the persistent launch that fit the device at once.
-hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid);
+hipError_t batch_main_grid(int dt, size_t lds, int32_t *grid, bool sv = false);
 
 }  // namespace segtaco

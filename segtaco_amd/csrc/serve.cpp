@@ -26,6 +26,7 @@
 
 #include "engine.h"
 #include "launch.h"
+#include "spmv.h"
 #include "segtaco/autotune.h"
 #include "segtaco/serve.h"
 
@@ -190,7 +191,7 @@ struct segtaco_server {
   std::thread th;
   bool stop = false;
   int64_t next_id = 0, use_clock = 0;
-  std::map<size_t, int32_t> grid_by_lds;
+  std::map<std::pair<size_t, bool>, int32_t> grid_by_lds;  // (LDS, N = 1 form)
   // statistics
   std::vector<double> lat_us;
   int64_t submitted = 0, completed = 0, rejected = 0, ticks = 0;
@@ -397,6 +398,7 @@ void launch_tick(segtaco_server *s, int32_t si) {
   }
   if (chosen.empty() && !issued) return;
   size_t lds = 0;
+  bool sv = false;  // some product runs the N = 1 form of eb_pr
   int64_t total = 0, total_fix = 0;
   slot.h_prefix[0] = slot.h_fix[0] = 0;
   for (size_t i = 0; i < chosen.size(); ++i) {
@@ -423,15 +425,16 @@ void launch_tick(segtaco_server *s, int32_t si) {
     slot.h_prefix[i + 1] = total;
     slot.h_fix[i + 1] = total_fix;
     lds = std::max(lds, l);
+    sv = sv || batch_uses_spmv(D);
     r.flops = 2.0 * m.nnz * r.N;
   }
   const int32_t n = (int32_t)chosen.size();
   int32_t grid = 1;
-  auto g = s->grid_by_lds.find(lds);
+  auto g = s->grid_by_lds.find({lds, sv});
   if (g != s->grid_by_lds.end()) {
     grid = g->second;
-  } else if (batch_main_grid(s->cfg.dtype, lds, &grid) == hipSuccess) {
-    s->grid_by_lds[lds] = grid;
+  } else if (batch_main_grid(s->cfg.dtype, lds, &grid, sv) == hipSuccess) {
+    s->grid_by_lds[{lds, sv}] = grid;
   } else {
     grid = 1024;
   }
@@ -444,7 +447,7 @@ void launch_tick(segtaco_server *s, int32_t si) {
                        hipMemcpyHostToDevice, slot.stream);
   if (e == hipSuccess && n > 0)
     e = launch_batch_main(s->cfg.dtype, slot.d_desc, n, slot.d_prefix, total,
-                          slot.d_counter, grid, lds, slot.stream, /*reset_counter=*/false);
+                          slot.d_counter, grid, lds, slot.stream, /*reset_counter=*/false, sv);
   if (e == hipSuccess && n > 0)
     e = launch_batch_fixup(slot.d_desc, n, slot.d_fix, total_fix, slot.stream);
   if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);

@@ -88,3 +88,58 @@ def test_random_structures_plans_and_shapes(seed):
     Br = B if dtype == "f32" else O.round_to(dtype, B)[1]
     err = O.scaled_err(Cg, O.spmm_f64(pos, crd, vr, Br), O.spmm_absbound(pos, crd, vr, Br))
     assert err <= 1e-5, f"seed {seed}: scaled error {err:.3e}"
+
+
+@pytest.mark.parametrize("seed", list(range(12)))
+def test_random_batches(seed):
+    """Random batches of 4-16 products (random structures, widths, padded
+    leading dimensions; the autotuner's plan or a random rb/eb plan per
+    product) in one persistent launch: every product within tolerance of the
+    FP64 oracle and bit-identical to the same plan run alone."""
+    from gpu_util import stream_ptr, torch
+    t = torch()
+    rng = np.random.default_rng(zlib.crc32(f"batchfuzz{seed}".encode()))
+    dtype = str(rng.choice(["f32", "bf16", "fp16"]))
+    n = int(rng.integers(4, 17))
+    prods, items, keep = [], [], []
+    for i in range(n):
+        pos, crd, vals, K = _structure(rng)
+        N = int(rng.choice([1, 3, 16, 32, 64, 100, 128]))
+        ldb = N + int(rng.choice([0, 0, 4]))
+        ldc = N + int(rng.choice([0, 0, 3]))
+        if rng.random() < 0.5:
+            plan = None
+        else:
+            name, kw = _plan(rng, "f32")
+            if name == "eb_cs":
+                name, kw = "eb_sr", {}
+            plan = S.make_plan(name, **kw)
+        A = DeviceCSR(pos, crd, vals, K, dtype=dtype)
+        B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        Bp = np.zeros((K, ldb), np.float32)
+        Bp[:, :N] = B
+        Bd = dense_dev(Bp, dtype)
+        C = t.full((max(A.M, 1), ldc), float("nan"), dtype=t.float32, device="cuda")
+        prods.append((pos, crd, vals, B, N, ldb, ldc))
+        keep.append((A, Bd, C))
+        items.append(S.batch_item(A.desc, Bd.data_ptr(), ldb, C.data_ptr(), ldc, N, plan,
+                                  np.ascontiguousarray(pos, np.int32)))
+    print(f"[batch fuzz] seed {seed}: {n} products {dtype}", flush=True)
+    try:
+        batch = S.Batch(items, dtype, stream_ptr())
+    except S.SegtacoError as e:
+        pytest.skip(f"batch refused: {e}")
+    batch.run(stream_ptr())
+    t.cuda.synchronize()
+    for i, ((pos, crd, vals, B, N, ldb, ldc), (A, Bd, C)) in enumerate(zip(prods, keep)):
+        M = len(pos) - 1
+        Cg = C[:M, :N].cpu().numpy()
+        vr = vals if dtype == "f32" else O.round_to(dtype, vals)[1]
+        Br = B if dtype == "f32" else O.round_to(dtype, B)[1]
+        err = O.scaled_err(Cg, O.spmm_f64(pos, crd, vr, Br), O.spmm_absbound(pos, crd, vr, Br))
+        plan = batch.plan(i)
+        assert err <= 1e-5, f"seed {seed} product {i} plan {plan.as_dict()}: {err:.3e}"
+        if M:
+            alone = run_spmm(A, Bd, N, plan, ldb=ldb, ldc=ldc, b_dtype=dtype)
+            assert np.array_equal(alone[:M, :N].cpu().numpy(), Cg), \
+                f"seed {seed} product {i}: batch != single call"
