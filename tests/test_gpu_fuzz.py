@@ -143,3 +143,30 @@ def test_random_batches(seed):
             alone = run_spmm(A, Bd, N, plan, ldb=ldb, ldc=ldc, b_dtype=dtype)
             assert np.array_equal(alone[:M, :N].cpu().numpy(), Cg), \
                 f"seed {seed} product {i}: batch != single call"
+
+
+@pytest.mark.parametrize("seed", list(range(24)))
+def test_random_sddmm(seed):
+    """SDDMM over random structures, storage types, widths J and padded
+    leading dimensions of X and Y, with and without values: within 1e-5 of
+    the FP64 oracle (16-bit storage: on the rounded inputs)."""
+    from test_sddmm import TOL, _err, _run
+    rng = np.random.default_rng(zlib.crc32(f"sddmmfuzz{seed}".encode()))
+    pos, crd, vals, K = _structure(rng)
+    M = len(pos) - 1
+    dtype = str(rng.choice(["f32", "bf16", "fp16"]))
+    J = int(rng.choice([1, 2, 5, 8, 16, 37, 64, 128, 200, 300]))
+    ldx = J + int(rng.choice([0, 0, 1, 8]))
+    ldy = J + int(rng.choice([0, 0, 3, 8]))
+    v = vals if rng.random() < 0.85 else None
+    X = rng.uniform(-1, 1, (M, J)).astype(np.float32)
+    Y = rng.uniform(-1, 1, (K, J)).astype(np.float32)
+    if dtype != "f32":
+        X, Y = O.round_to(dtype, X)[1], O.round_to(dtype, Y)[1]
+        v = None if v is None else O.round_to(dtype, v)[1]
+    print(f"[sddmm fuzz] seed {seed}: M={M} K={K} nnz={len(crd)} {dtype} J={J} "
+          f"ldx={ldx} ldy={ldy} vals={v is not None}", flush=True)
+    out = _run(pos, crd, v, X, Y, dtype, ldx=ldx, ldy=ldy)[0]
+    ref = O.sddmm_f64(pos, crd, v, X, Y)
+    scale = O.sddmm_f64(pos, crd, v, X, Y, absval=True)
+    assert _err(out, ref, scale) <= TOL, f"seed {seed}"
