@@ -170,3 +170,18 @@ def test_random_sddmm(seed):
     ref = O.sddmm_f64(pos, crd, v, X, Y)
     scale = O.sddmm_f64(pos, crd, v, X, Y, absval=True)
     assert _err(out, ref, scale) <= TOL, f"seed {seed}"
+
+
+def test_random_operator_surface():
+    """Every one of the 79 operator-surface names (host operands in, C out,
+    the reference's calling convention) on its own random structure and a
+    random width from 1 to 300: within 1e-5 of the FP64 oracle."""
+    for name in S.legacy_names():
+        rng = np.random.default_rng(zlib.crc32(f"legacy{name}".encode()))
+        pos, crd, vals, K = _structure(rng)
+        N = int(rng.choice([1, 2, 4, 17, 32, 128, 130, 256, 300]))
+        B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        print(f"[legacy fuzz] {name}: M={len(pos) - 1} K={K} nnz={len(crd)} N={N}", flush=True)
+        C, _ = S.legacy(name, pos, crd, vals, B)
+        err = O.scaled_err(C, O.spmm_f64(pos, crd, vals, B), O.spmm_absbound(pos, crd, vals, B))
+        assert err <= 1e-5, f"{name}: scaled error {err:.3e}"
