@@ -19,7 +19,7 @@ from gpu_util import DeviceCSR, dense_dev, run_spmm
 
 pytestmark = pytest.mark.gpu
 
-SEEDS = list(range(48))
+SEEDS = list(range(96))
 
 
 def _structure(rng):
@@ -90,7 +90,7 @@ def test_random_structures_plans_and_shapes(seed):
     assert err <= 1e-5, f"seed {seed}: scaled error {err:.3e}"
 
 
-@pytest.mark.parametrize("seed", list(range(12)))
+@pytest.mark.parametrize("seed", list(range(24)))
 def test_random_batches(seed):
     """Random batches of 4-16 products (random structures, widths, padded
     leading dimensions; the autotuner's plan or a random rb/eb plan per
