@@ -185,3 +185,47 @@ def test_random_operator_surface():
         C, _ = S.legacy(name, pos, crd, vals, B)
         err = O.scaled_err(C, O.spmm_f64(pos, crd, vals, B), O.spmm_absbound(pos, crd, vals, B))
         assert err <= 1e-5, f"{name}: scaled error {err:.3e}"
+
+
+@pytest.mark.parametrize("seed", list(range(32)))
+def test_random_banded_block_dense(seed):
+    """bd (32-row blocks on the matrix cores) on random banded structures —
+    band half-width 0-100 around a shifted diagonal, random density, M not a
+    multiple of 32, K from 1 — so that most blocks' column windows fit the
+    256-column limit and take the MFMA path; random widths and padded
+    leading dimensions, BF16 and FP16, with and without values."""
+    rng = np.random.default_rng(zlib.crc32(f"bdfuzz{seed}".encode()))
+    M = int(rng.integers(1, 3000))
+    K = int(rng.integers(1, 3000))
+    hw = int(rng.integers(0, 101))
+    shift = int(rng.integers(-50, 51))
+    dens = float(rng.uniform(0.05, 1.0))
+    rows = []
+    for i in range(M):
+        c = int(round(i * K / max(M, 1))) + shift
+        lo, hi = max(0, c - hw), min(K, c + hw + 1)
+        if hi <= lo:
+            rows.append(np.zeros(0, np.int64))
+            continue
+        cols = np.arange(lo, hi)
+        rows.append(cols[rng.random(len(cols)) < dens])
+    pos = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    crd = np.concatenate(rows + [np.zeros(0, np.int64)]).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(crd)).astype(np.float32)
+    dtype = str(rng.choice(["bf16", "fp16"]))
+    N = int(rng.choice([1, 7, 32, 64, 100, 128, 200, 256, 300]))
+    ldb = N + int(rng.choice([0, 0, 5, 8]))
+    ldc = N + int(rng.choice([0, 0, 3]))
+    v = vals if rng.random() < 0.85 else None
+    print(f"[bd fuzz] seed {seed}: M={M} K={K} nnz={len(crd)} hw={hw} {dtype} N={N} "
+          f"ldb={ldb} ldc={ldc} vals={v is not None}", flush=True)
+    A = DeviceCSR(pos, crd, v, K, dtype=dtype)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    Bp = np.zeros((K, ldb), np.float32)
+    Bp[:, :N] = B
+    C = run_spmm(A, dense_dev(Bp, dtype), N, S.make_plan("bd"), ldb=ldb, ldc=ldc, b_dtype=dtype)
+    Cg = C[:M, :N].cpu().numpy()
+    vr = None if v is None else O.round_to(dtype, v)[1]
+    Br = O.round_to(dtype, B)[1]
+    err = O.scaled_err(Cg, O.spmm_f64(pos, crd, vr, Br), O.spmm_absbound(pos, crd, vr, Br))
+    assert err <= 1e-5, f"seed {seed}: scaled error {err:.3e}"
