@@ -32,21 +32,44 @@ COMMON = [
     "-Wall", "-Wno-unused-command-line-argument", "-Wno-unused-function",
 ]
 
-LIB_SOURCES = [
-    "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
-    "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp", "io.cpp", "vendor.cpp",
-    "k_sddmm.hip", "sddmm.cpp", "k_cs.hip", "k_spmv.hip", "k_bd.hip", "staging.cpp",
-]
+# (source, object name, extra flags): the batched and element-balanced
+# kernels are compiled once per storage type (and, for the batch, per N = 1
+# form), so those objects build in parallel instead of in one long pass.
+LIB_SOURCES = (
+    [(s, os.path.splitext(s)[0], []) for s in (
+        "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
+        "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp", "io.cpp", "vendor.cpp",
+        "k_sddmm.hip", "sddmm.cpp", "k_cs.hip", "k_spmv.hip", "k_bd.hip", "staging.cpp")]
+    + [("k_batch_inst.hip", f"k_batch_{dt}_{sv}",
+        [f"-DSEGTACO_BATCH_DT={dt}", f"-DSEGTACO_BATCH_SV={sv}"])
+       for dt in range(3) for sv in range(2)]
+    + [("k_eb_inst.hip", f"k_eb_{dt}", [f"-DSEGTACO_EB_DT={dt}"]) for dt in range(3)]
+)
 # rocSPARSE backs the vendor comparison row only (vendor.cpp).
 LIB_LINK = ["-L/opt/rocm/lib", "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"]
 
 
-def _compile(src: str) -> str:
+def _stale(obj: str) -> bool:
+    """Whether obj is older than any source or header under csrc/ or include/
+    (used only by incremental developer builds)."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    for d in (CSRC, os.path.join(ROOT, "include", "segtaco")):
+        for f in os.listdir(d):
+            if os.path.getmtime(os.path.join(d, f)) > t:
+                return True
+    return False
+
+
+def _compile(entry, incremental: bool = False) -> str:
+    src, name, extra = entry
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
-    if src.endswith(".cpp"):
-        cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", path, "-o", obj]
+    obj = os.path.join(OBJ, name + ".o")
+    if incremental and not _stale(obj):
+        return obj
+    lang = ["-x", "hip"] if src.endswith(".cpp") else []
+    cmd = [HIPCC] + COMMON + extra + lang + ["-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
@@ -56,11 +79,16 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, incremental: bool = False) -> str:
+    """Compile every source and link libsegtaco.so and the driver.
+    incremental=True (developer use) recompiles only objects older than a
+    source or header; the default compiles everything."""
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(8, os.cpu_count() or 4)
+    # longest compiles first, so the pool is not left waiting on one of them
+    order = sorted(LIB_SOURCES, key=lambda e: not e[0].endswith("_inst.hip"))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, LIB_SOURCES))
+        objs = list(ex.map(lambda e: _compile(e, incremental), order))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + LIB_LINK
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -93,5 +121,5 @@ def build_oracle(verbose: bool = True) -> None:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     build_oracle()

@@ -1,27 +1,12 @@
-// k_eb.hip — standalone launches of the element-balanced schedules and their
-// fix-up (bodies in sched_eb.h), the K21 row partition and a zero fill.
+// k_eb.hip — host dispatch of the standalone element-balanced kernels (one
+// object per storage type, k_eb_inst.hip), their carry fix-up (bodies in
+// sched_eb.h), the K21 row partition, the tile-row scatter and a zero fill.
 #include "sched_eb.h"
 
 namespace segtaco {
 
 size_t eb_sr_lds_bytes(const KArgs &a) { return eb_sr_lds_bytes_lq(a.lq); }
 size_t eb_pr_lds_bytes(const KArgs &) { return eb_pr_lds_bytes_fixed(); }
-
-template <int DT, int V>
-__global__ __launch_bounds__(kBlockThreads) void k_eb_sr(const KArgs a) {
-  extern __shared__ int32_t smem[];
-  int32_t blk, panel;
-  if (!decode_block(a.panels, a.nblocks, blk, panel)) return;
-  eb_sr_body<DT, V>(a, blk, panel, smem);
-}
-
-template <int DT, int V>
-__global__ __launch_bounds__(kBlockThreads) void k_eb_pr(const KArgs a) {
-  extern __shared__ int32_t smem[];
-  int32_t blk, panel;
-  if (!decode_block(a.panels, a.nblocks, blk, panel)) return;
-  eb_pr_body<DT, V>(a, blk, panel, smem);
-}
 
 template <int V>
 __global__ __launch_bounds__(kBlockThreads) void k_fixup(const KArgs a) {
@@ -118,41 +103,22 @@ __global__ __launch_bounds__(kBlockThreads) void k_fill_zero(float *C,
   }
 }
 
-#define SEGTACO_DISPATCH_LDS(KERNEL, dt, vec, grid, lds, args, stream)         \
-  do {                                                                        \
-    const dim3 g_((unsigned)(grid)), b_(kBlockThreads);                       \
-    switch ((dt) * 16 + (vec)) {                                              \
-      case 0 * 16 + 1: hipLaunchKernelGGL((KERNEL<0, 1>), g_, b_, lds, stream, args); break; \
-      case 0 * 16 + 2: hipLaunchKernelGGL((KERNEL<0, 2>), g_, b_, lds, stream, args); break; \
-      case 0 * 16 + 4: hipLaunchKernelGGL((KERNEL<0, 4>), g_, b_, lds, stream, args); break; \
-      case 1 * 16 + 1: hipLaunchKernelGGL((KERNEL<1, 1>), g_, b_, lds, stream, args); break; \
-      case 1 * 16 + 2: hipLaunchKernelGGL((KERNEL<1, 2>), g_, b_, lds, stream, args); break; \
-      case 1 * 16 + 4: hipLaunchKernelGGL((KERNEL<1, 4>), g_, b_, lds, stream, args); break; \
-      case 1 * 16 + 8: hipLaunchKernelGGL((KERNEL<1, 8>), g_, b_, lds, stream, args); break; \
-      case 2 * 16 + 1: hipLaunchKernelGGL((KERNEL<2, 1>), g_, b_, lds, stream, args); break; \
-      case 2 * 16 + 2: hipLaunchKernelGGL((KERNEL<2, 2>), g_, b_, lds, stream, args); break; \
-      case 2 * 16 + 4: hipLaunchKernelGGL((KERNEL<2, 4>), g_, b_, lds, stream, args); break; \
-      case 2 * 16 + 8: hipLaunchKernelGGL((KERNEL<2, 8>), g_, b_, lds, stream, args); break; \
-      default: return hipErrorInvalidValue;                                   \
-    }                                                                         \
-  } while (0)
-
 hipError_t launch_eb_sr(int dt, int vec, const KArgs &a, hipStream_t s) {
-  const int64_t grid = grid_for(a.panels, a.nblocks);
-  if (grid <= 0) return hipSuccess;
-  if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
-  const size_t lds = eb_sr_lds_bytes(a);
-  SEGTACO_DISPATCH_LDS(k_eb_sr, dt, vec, grid, lds, a, s);
-  return hipGetLastError();
+  switch (dt) {
+    case 0: return launch_eb_sr_t<0>(vec, a, s);
+    case 1: return launch_eb_sr_t<1>(vec, a, s);
+    case 2: return launch_eb_sr_t<2>(vec, a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_eb_pr(int dt, int vec, const KArgs &a, hipStream_t s) {
-  const int64_t grid = grid_for(a.panels, a.nblocks);
-  if (grid <= 0) return hipSuccess;
-  if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
-  const size_t lds = eb_pr_lds_bytes(a);
-  SEGTACO_DISPATCH_LDS(k_eb_pr, dt, vec, grid, lds, a, s);
-  return hipGetLastError();
+  switch (dt) {
+    case 0: return launch_eb_pr_t<0>(vec, a, s);
+    case 1: return launch_eb_pr_t<1>(vec, a, s);
+    case 2: return launch_eb_pr_t<2>(vec, a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_fixup(int vec, const KArgs &a, hipStream_t s) {

@@ -53,6 +53,11 @@ hipError_t launch_rb_sr(int dt, int vec, const KArgs &a, hipStream_t s);
 hipError_t launch_rb_pr(int dt, int vec, const KArgs &a, hipStream_t s);
 hipError_t launch_eb_sr(int dt, int vec, const KArgs &a, hipStream_t s);
 hipError_t launch_eb_pr(int dt, int vec, const KArgs &a, hipStream_t s);
+// The same for one storage type DT (k_eb_inst.hip, one object per DT).
+template <int DT>
+hipError_t launch_eb_sr_t(int vec, const KArgs &a, hipStream_t s);
+template <int DT>
+hipError_t launch_eb_pr_t(int vec, const KArgs &a, hipStream_t s);
 // Combines carry partials of rows that span eb tiles (vec in floats).
 hipError_t launch_fixup(int vec, const KArgs &a, hipStream_t s);
 // out[b] = K21 search for b*T (bit-exact); rows (optional) gets the row that
