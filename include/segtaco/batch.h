@@ -55,6 +55,12 @@ int32_t segtaco_batch_replay(segtaco_batch_t b, void *stream);
  * bytes of device workspace the batch holds. */
 int32_t segtaco_batch_plan(segtaco_batch_t b, int32_t i, segtaco_plan_t *out);
 int64_t segtaco_batch_units(segtaco_batch_t b);
+/* Item i's row partition as the batch holds it (device pointers, valid until
+ * the batch is destroyed): the K21 array (gpu_library.h:206-215) in *part and
+ * the repaired first row of each tile in *tile_rows (ntiles + 1 entries
+ * each); *ntiles = 0 and NULL pointers for row-balanced items. */
+int32_t segtaco_batch_partition(segtaco_batch_t b, int32_t i, int32_t *ntiles,
+                                const int32_t **part, const int32_t **tile_rows);
 size_t segtaco_batch_workspace_bytes(segtaco_batch_t b);
 int32_t segtaco_batch_destroy(segtaco_batch_t b);
 

@@ -234,7 +234,9 @@ typedef struct segtaco_csr {
 int32_t segtaco_plan_resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
                              int64_t ldb, int64_t ldc, segtaco_plan_t *plan);
 
-/* Workspace bytes segtaco_spmm needs for this (resolved) plan. */
+/* Workspace bytes segtaco_spmm needs for this plan: exact for a resolved
+ * plan; for a plan with open (zero) fields, enough for every resolution a
+ * call with these operands can reach. */
 size_t segtaco_spmm_workspace_bytes(const segtaco_csr_t *A, int32_t N,
                                     const segtaco_plan_t *plan);
 
@@ -263,6 +265,42 @@ int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,
 int32_t segtaco_block_starts(const int32_t *pos, int32_t M,
                              int32_t values_per_block, int32_t num_blocks,
                              int32_t *out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Partition introspection: the row partitions the schedules run on          */
+/* ------------------------------------------------------------------------ */
+#define SEGTACO_ROWS_SCATTER 0 /* one pass over pos (standalone calls, SDDMM) */
+#define SEGTACO_ROWS_SEARCH 1  /* K21 search + repair per tile (batches, server) */
+
+/* rows[b] = the row holding stored entry b*tile for b < ntiles =
+ * ceil(nnz / tile), rows[ntiles] = M: the K21 search (gpu_library.h:185-215)
+ * followed by the repair loop that steps over empty rows (spmm_eb_sr.h:
+ * 39-42), i.e. the row each element-balanced tile starts at. With
+ * SEGTACO_ROWS_SEARCH and part != NULL, part gets the unrepaired K21 array
+ * (as segtaco_block_starts). Device pointers; asynchronous on `stream`. */
+int32_t segtaco_tile_rows(const int32_t *pos, int32_t M, int32_t tile, int32_t ntiles,
+                          int32_t method, int32_t *rows, int32_t *part, void *stream);
+
+/* Where a call of segtaco_spmm with this resolved element-balanced plan
+ * (eb_sr, eb_pr, eb_cs) keeps its partition in the workspace: out[0] =
+ * ntiles; out[1] = byte offset of the per-tile fix-up records (int32
+ * [ntiles]: the tile where the row ending in tile b started if it started
+ * in an earlier tile, else -1); out[2] = byte offset of the tile rows (int32
+ * [ntiles + 1], as segtaco_tile_rows); out[3], out[4] = byte offsets of the
+ * carry arrays (float [ntiles][N] each: partial sums of rows continued from
+ * / into a neighbouring tile). */
+int32_t segtaco_spmm_workspace_layout(const segtaco_csr_t *A, int32_t N,
+                                      const segtaco_plan_t *plan, int64_t *out);
+
+/* rows[p] for every stored entry p < nnz: its row as the eb_pr schedule
+ * derives it inside tiles of `tile` entries (per 128-entry chunk, a
+ * bisection in a staged window of the row pointer, continued in memory for
+ * rows past the window) — the segment ids of its segmented reduction (the
+ * reference computes them with binary_search_segment_number,
+ * gpu_library.h:138-152). tile_rows as from segtaco_tile_rows. Device
+ * pointers; asynchronous on `stream`. */
+int32_t segtaco_entry_rows(const int32_t *pos, int32_t M, int32_t nnz, int32_t tile,
+                           const int32_t *tile_rows, int32_t *rows, void *stream);
 
 /* Host-side CSR structure check: pos[0] == 0, pos non-decreasing,
  * pos[M] == nnz, every crd in [0, K). Host pointers. */

@@ -54,6 +54,10 @@ def lib():
         L.or_segment_number.argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32]
         L.or_row_of_nnz.restype = None
         L.or_row_of_nnz.argtypes = [_i32p, ctypes.c_int32, _i32p]
+        for fn in ("or_tile_rows", "or_fix_t0"):
+            getattr(L, fn).restype = None
+            getattr(L, fn).argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       _i32p]
         L.or_spmm_f32_seq.restype = None
         L.or_spmm_f32_seq.argtypes = [ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _f32p,
                                       _f32p, ctypes.c_int64, _f32p, ctypes.c_int64]
@@ -135,10 +139,31 @@ def segment_number(pos: np.ndarray, M: int, elem: int) -> int:
 
 
 def row_of_nnz(pos: np.ndarray, M: int) -> np.ndarray:
+    """Row of every stored entry (the COO row array)."""
     pos = np.ascontiguousarray(pos, dtype=np.int32)
     rows = np.empty(max(int(pos[M]), 1), dtype=np.int32)
     lib().or_row_of_nnz(_ptr(pos, _i32p), M, _ptr(rows, _i32p))
     return rows[: int(pos[M])]
+
+
+def tile_rows(pos: np.ndarray, M: int, T: int) -> np.ndarray:
+    """First row of each of the ceil(nnz/T) element-balanced tiles: the K21
+    search plus the repair loop of spmm_eb_sr.h:39-42; entry ntiles is M."""
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    nt = (int(pos[M]) + T - 1) // T
+    out = np.empty(nt + 1, dtype=np.int32)
+    lib().or_tile_rows(_ptr(pos, _i32p), M, T, nt, _ptr(out, _i32p))
+    return out
+
+
+def fix_t0(pos: np.ndarray, M: int, T: int) -> np.ndarray:
+    """Per tile: the tile where the row ending in it started, if that row
+    started in an earlier tile, else -1."""
+    pos = np.ascontiguousarray(pos, dtype=np.int32)
+    nt = (int(pos[M]) + T - 1) // T
+    out = np.empty(max(nt, 1), dtype=np.int32)
+    lib().or_fix_t0(_ptr(pos, _i32p), M, T, nt, _ptr(out, _i32p))
+    return out[:nt]
 
 
 # ----------------------------------------------------------------------------

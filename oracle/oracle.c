@@ -69,6 +69,41 @@ void or_row_of_nnz(const int32_t *pos, int32_t M, int32_t *rows) {
     for (int32_t p = pos[r]; p < pos[r + 1]; ++p) rows[p] = r;
 }
 
+/* First row of every element-balanced tile, as the reference's tiled
+ * kernels reach it: the K21 search (taco_binarySearchBefore for b*T,
+ * gpu_library.h:185-205) followed by the repair loop that steps over the
+ * empty rows sharing that row pointer (spmm_eb_sr.h:39-42:
+ * while (fposA == A2_pos[i_pos + 1]) i_pos++). rows[b] for b < ntiles is the
+ * row holding entry b*T; rows[ntiles] = M (the search's early return for a
+ * target at or past nnz). ntiles = ceil(nnz / T). */
+void or_tile_rows(const int32_t *pos, int32_t M, int32_t T, int32_t ntiles,
+                  int32_t *rows) {
+  for (int32_t b = 0; b <= ntiles; ++b) {
+    const int64_t target64 = (int64_t)b * T;
+    const int32_t target = target64 > pos[M] ? pos[M] : (int32_t)target64;
+    int32_t r = or_search_before(pos, 0, M, target);
+    if (b < ntiles)
+      while (pos[r + 1] == target) ++r;
+    rows[b] = r;
+  }
+}
+
+/* Carry bookkeeping of a standalone element-balanced call: fix[b] = the tile
+ * in which the row holding tile b's first entry started, when that row began
+ * before tile b and ends inside tile b (its carries are then combined by the
+ * fix-up of tile b); -1 otherwise. Walks the rows: a non-empty row with
+ * entries [p0, p1) spans tiles p0/T .. (p1-1)/T. */
+void or_fix_t0(const int32_t *pos, int32_t M, int32_t T, int32_t ntiles,
+               int32_t *fix) {
+  for (int32_t b = 0; b < ntiles; ++b) fix[b] = -1;
+  for (int32_t r = 0; r < M; ++r) {
+    const int32_t p0 = pos[r], p1 = pos[r + 1];
+    if (p0 == p1) continue;
+    const int32_t t0 = p0 / T, t1 = (p1 - 1) / T;
+    if (t0 < t1) fix[t1] = t0;
+  }
+}
+
 /* Restates Eigen 3.4.0 sparse_time_dense_product_impl<..., RowMajor>::
  * processRow (SparseDenseProduct.h:141-146) under C.noalias() = A * B:
  * C is zeroed, then for each row, for each stored entry in CSR order,

@@ -11,6 +11,15 @@
  *                         invariants + exhaustive tie-break enumeration in tests.
  *   or_block_starts       restates gpu_library.h:206-215,236-240; pinned as above.
  *   or_segment_number     restates gpu_library.h:138-152; pinned by brute force.
+ *   or_row_of_nnz         per-entry row ids (the COO row array); pinned against
+ *                         numpy's repeat(arange(M), diff(pos)) and against
+ *                         or_segment_number at every entry.
+ *   or_tile_rows          K21 search + the repair loop of spmm_eb_sr.h:39-42
+ *                         (the row each element-balanced tile starts at);
+ *                         pinned against the row holding entry b*T (brute
+ *                         force) and by a planted repair-less variant.
+ *   or_fix_t0             the fix-up's per-tile record (this build's carry
+ *                         bookkeeping); pinned by a tile-wise brute force.
  *   or_spmm_f32_seq       restates Eigen 3.4.0 SparseDenseProduct.h:109-147
  *                         (row-major CSR x row-major dense); pinned bit-exactly
  *                         against the real Eigen product (oracle/_ref) and the
@@ -18,7 +27,10 @@
  *   or_spmm_f64           closed form sum_p v_p * B[crd_p, k] in double; pinned
  *                         by special cases (A = I, B = I, B = ones, N = 1) and a
  *                         dense brute-force product.
- *   or_spmm_absbound      sum_p |v_p| * |B[crd_p, k]| in double (error scale).
+ *   or_spmm_absbound      sum_p |v_p| * |B[crd_p, k]| in double (error scale);
+ *                         pinned against the dense |A| |B| product in float64
+ *                         (empty rows, no values), planted x2 / signed variants
+ *                         rejected.
  *   or_sddmm_f64          SDDMM closed form in double (sddmm_csr_gpu_taco.h:
  *                         70-148 computes it in FP64); pinned by a dense
  *                         brute-force (X Y^T masked by the pattern) and special
@@ -42,6 +54,10 @@ void or_block_starts(const int32_t *pos, int32_t M, int32_t values_per_block,
 int32_t or_segment_number(const int32_t *seg_offsets, int32_t n_seg,
                           int32_t elem_id);
 void or_row_of_nnz(const int32_t *pos, int32_t M, int32_t *rows);
+void or_tile_rows(const int32_t *pos, int32_t M, int32_t T, int32_t ntiles,
+                  int32_t *rows);
+void or_fix_t0(const int32_t *pos, int32_t M, int32_t T, int32_t ntiles,
+               int32_t *fix);
 
 void or_spmm_f32_seq(int32_t M, int32_t N, const int32_t *pos,
                      const int32_t *crd, const float *vals, const float *B,

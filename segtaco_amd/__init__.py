@@ -181,6 +181,19 @@ def lib() -> ctypes.CDLL:
     L.segtaco_block_starts.restype = ctypes.c_int32
     L.segtaco_block_starts.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        _vp, _vp]
+    L.segtaco_tile_rows.restype = ctypes.c_int32
+    L.segtaco_tile_rows.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, _vp, _vp, _vp]
+    L.segtaco_spmm_workspace_layout.restype = ctypes.c_int32
+    L.segtaco_spmm_workspace_layout.argtypes = [ctypes.POINTER(CSR), ctypes.c_int32,
+                                                ctypes.POINTER(Plan),
+                                                ctypes.POINTER(ctypes.c_int64)]
+    L.segtaco_entry_rows.restype = ctypes.c_int32
+    L.segtaco_entry_rows.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+                                     _vp, _vp]
+    L.segtaco_batch_partition.restype = ctypes.c_int32
+    L.segtaco_batch_partition.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
     L.segtaco_csr_validate_host.restype = ctypes.c_int32
     L.segtaco_csr_validate_host.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             _i32p, _i32p]
@@ -615,6 +628,42 @@ def block_starts(pos_ptr: int, M: int, T: int, num_blocks: int, out_ptr: int,
     check(lib().segtaco_block_starts(pos_ptr, M, T, num_blocks, out_ptr, stream or None))
 
 
+ROWS_SCATTER = 0
+ROWS_SEARCH = 1
+
+
+def tile_rows(pos_ptr: int, M: int, T: int, ntiles: int, rows_ptr: int, method: int,
+              part_ptr: int | None = None, stream: int | None = None) -> None:
+    """First row of every element-balanced tile (K21 search + repair), on
+    device; method ROWS_SCATTER (standalone calls, SDDMM) or ROWS_SEARCH
+    (batches, server; also the K21 array into part_ptr)."""
+    check(lib().segtaco_tile_rows(pos_ptr, M, T, ntiles, method, rows_ptr, part_ptr or None,
+                                  stream or None))
+
+
+def workspace_layout(A: CSR, N: int, plan: Plan) -> dict:
+    """Where an element-balanced call keeps its partition in the workspace."""
+    out = (ctypes.c_int64 * 5)()
+    check(lib().segtaco_spmm_workspace_layout(ctypes.byref(A), N, ctypes.byref(plan), out))
+    return dict(ntiles=int(out[0]), fix_off=int(out[1]), rows_off=int(out[2]),
+                head_off=int(out[3]), tail_off=int(out[4]))
+
+
+def entry_rows(pos_ptr: int, M: int, nnz: int, T: int, tile_rows_ptr: int, rows_ptr: int,
+               stream: int | None = None) -> None:
+    """Row of every stored entry as eb_pr's walk derives it, on device."""
+    check(lib().segtaco_entry_rows(pos_ptr, M, nnz, T, tile_rows_ptr, rows_ptr,
+                                   stream or None))
+
+
+def device_to_host(ptr: int, n: int, dtype=np.int32) -> np.ndarray:
+    """Synchronous copy of n elements from device memory."""
+    out = np.empty(n, dtype=dtype)
+    if n:
+        check(lib().segtaco_memcpy(out.ctypes.data, ptr, out.nbytes, 1))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Autotuner (include/segtaco/autotune.h)
 # ---------------------------------------------------------------------------
@@ -734,6 +783,18 @@ class Batch:
 
     def workspace_bytes(self) -> int:
         return int(lib().segtaco_batch_workspace_bytes(self.handle))
+
+    def partition(self, i: int):
+        """Item i's (K21 array, tile rows) as the batch holds them, copied to
+        the host; (None, None) for a row-balanced item."""
+        nt = ctypes.c_int32()
+        part, rows = _vp(), _vp()
+        check(lib().segtaco_batch_partition(self.handle, i, ctypes.byref(nt), ctypes.byref(part),
+                                            ctypes.byref(rows)))
+        if nt.value == 0:
+            return None, None
+        return (device_to_host(part.value, nt.value + 1),
+                device_to_host(rows.value, nt.value + 1))
 
     def destroy(self) -> None:
         if self.handle:

@@ -322,6 +322,23 @@ int32_t segtaco_batch_plan(segtaco_batch_t b, int32_t i, segtaco_plan_t *out) {
   return SEGTACO_OK;
 }
 
+int32_t segtaco_batch_partition(segtaco_batch_t b, int32_t i, int32_t *ntiles,
+                                const int32_t **part, const int32_t **tile_rows) {
+  if (!b || !ntiles || !part || !tile_rows || i < 0 || i >= b->n)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_batch_partition: bad argument");
+  const BatchDesc &D = b->descs[i];
+  if (D.schedule < SEGTACO_EB_SR || D.a.ntiles == 0) {
+    *ntiles = 0;
+    *part = nullptr;
+    *tile_rows = nullptr;
+    return SEGTACO_OK;
+  }
+  *ntiles = D.a.ntiles;
+  *part = D.a.part;
+  *tile_rows = D.a.tile_row;
+  return SEGTACO_OK;
+}
+
 int64_t segtaco_batch_units(segtaco_batch_t b) { return b ? b->total_units : -1; }
 
 size_t segtaco_batch_workspace_bytes(segtaco_batch_t b) {

@@ -294,19 +294,67 @@ int32_t check_csr(const segtaco_csr_t *A) {
 
 }  // namespace
 
+namespace {
+
+// Workspace of a plan whose tile, group and vec are all set.
+size_t resolved_workspace_bytes(const segtaco_csr_t *A, int32_t N,
+                                const segtaco_plan_t &p, int32_t b_dtype) {
+  if (p.schedule == SEGTACO_EB_CS)
+    return cs_layout(A->nnz, N, p.tile, p.group, p.vec, elem_bytes(b_dtype),
+                     A->vals != nullptr)
+        .total;
+  if (p.schedule != SEGTACO_EB_SR && p.schedule != SEGTACO_EB_PR) return 0;
+  return eb_layout(A->nnz, N, p.tile).total;
+}
+
+}  // namespace
+
+// A resolved plan gives the exact bytes. For a plan with open fields the
+// answer covers every resolution a later call can reach: the plan is
+// resolved as segtaco_plan_resolve does (no pointers, ld = N), for each
+// storage type the call may use (the values' type, or FP32 and BF16 without
+// values — a wider vector means fewer lanes per row and smaller default
+// tiles, so more tiles), with eb_cs's open vector width taken as 1 (the most
+// rows per unit); the largest result is returned (0 when none resolves).
 size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
                                  const segtaco_plan_t *plan) {
-  if (!A || !plan || A->nnz == 0 || A->M == 0) return 0;
-  if (plan->schedule == SEGTACO_EB_CS) {
-    if (!plan->tile || !plan->group || !plan->vec) return 0;  // unresolved
-    return cs_layout(A->nnz, N, plan->tile, plan->group, plan->vec,
-                     elem_bytes(A->vals_dtype), A->vals != nullptr)
-        .total;
+  if (!A || !plan || A->nnz == 0 || A->M == 0 || N <= 0) return 0;
+  if (plan->schedule < SEGTACO_EB_SR || plan->schedule > SEGTACO_EB_CS) return 0;
+  const bool open = plan->tile == 0 || plan->group == 0 || plan->vec == 0 ||
+                    plan->col_lanes == 0 || plan->comp_threshold == -1;
+  const int32_t own_dt = A->vals ? A->vals_dtype : SEGTACO_DT_F32;
+  if (!open) return resolved_workspace_bytes(A, N, *plan, own_dt);
+  const int32_t dts[2] = {own_dt, SEGTACO_DT_BF16};
+  const int ndt = A->vals ? 1 : 2;
+  size_t best = 0;
+  for (int k = 0; k < ndt; ++k) {
+    segtaco_plan_t p = *plan;
+    if (p.schedule == SEGTACO_EB_CS && p.vec == 0) p.vec = 1;
+    if (resolve(A, N, dts[k], N, N, nullptr, nullptr, &p) != SEGTACO_OK) continue;
+    const size_t b = resolved_workspace_bytes(A, N, p, dts[k]);
+    if (b > best) best = b;
   }
-  if (plan->schedule != SEGTACO_EB_SR && plan->schedule != SEGTACO_EB_PR)
-    return 0;
-  const int32_t tile = plan->tile ? plan->tile : 128;
-  return eb_layout(A->nnz, N, tile).total;
+  segtaco_clear_error();
+  return best;
+}
+
+int32_t spmm_workspace_layout_impl(const segtaco_csr_t *A, int32_t N,
+                                   const segtaco_plan_t *plan, int64_t *out) {
+  if (!A || !plan || !out)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_spmm_workspace_layout: null argument");
+  if (plan->schedule < SEGTACO_EB_SR || plan->schedule > SEGTACO_EB_CS)
+    return set_error(SEGTACO_ERR_UNSUPPORTED,
+                     "segtaco_spmm_workspace_layout: schedule %d has no workspace",
+                     plan->schedule);
+  if (plan->tile < 8)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_spmm_workspace_layout: unresolved plan");
+  const EbLayout L = eb_layout(A->nnz, N, plan->tile);
+  out[0] = L.ntiles;
+  out[1] = (int64_t)L.part_off;
+  out[2] = (int64_t)L.rows_off;
+  out[3] = (int64_t)L.head_off;
+  out[4] = (int64_t)L.tail_off;
+  return SEGTACO_OK;
 }
 
 // ev (optional, 4 events): recorded before the partition, after it, after
@@ -522,6 +570,37 @@ int32_t segtaco_plan_resolve(const segtaco_csr_t *A, int32_t N,
 size_t segtaco_spmm_workspace_bytes(const segtaco_csr_t *A, int32_t N,
                                     const segtaco_plan_t *plan) {
   return spmm_workspace_bytes_impl(A, N, plan);
+}
+
+int32_t segtaco_spmm_workspace_layout(const segtaco_csr_t *A, int32_t N,
+                                      const segtaco_plan_t *plan, int64_t *out) {
+  return spmm_workspace_layout_impl(A, N, plan, out);
+}
+
+int32_t segtaco_tile_rows(const int32_t *pos, int32_t M, int32_t tile, int32_t ntiles,
+                          int32_t method, int32_t *rows, int32_t *part, void *stream) {
+  if (!pos || !rows || M < 0 || tile < 1 || ntiles < 0)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_tile_rows: bad argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (method == SEGTACO_ROWS_SCATTER) {
+    if (part) return set_error(SEGTACO_ERR_INVALID, "segtaco_tile_rows: part needs SEARCH");
+    SEGTACO_HIP(launch_tile_rows(pos, M, tile, ntiles, rows, s));
+  } else if (method == SEGTACO_ROWS_SEARCH) {
+    SEGTACO_HIP(launch_block_starts(pos, M, tile, ntiles, part, rows, s));
+  } else {
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_tile_rows: method %d", method);
+  }
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_entry_rows(const int32_t *pos, int32_t M, int32_t nnz, int32_t tile,
+                           const int32_t *tile_rows, int32_t *rows, void *stream) {
+  if (M < 0 || nnz < 0 || tile < 8 || (nnz > 0 && (!pos || !tile_rows || !rows)))
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_entry_rows: bad argument");
+  if (nnz == 0) return SEGTACO_OK;
+  SEGTACO_HIP(launch_entry_rows(pos, M, nnz, tile, tile_rows, rows,
+                                static_cast<hipStream_t>(stream)));
+  return SEGTACO_OK;
 }
 
 int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,

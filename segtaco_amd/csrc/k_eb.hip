@@ -91,6 +91,39 @@ __global__ __launch_bounds__(kBlockThreads) void k_tile_rows_fix(
   }
 }
 
+// The row of every stored entry as eb_pr's walk derives it (one wave per
+// tile, the same chunking, row-pointer window and bisection as
+// eb_pr_walk), written to rows[p]: exposes those per-entry row ids so they
+// can be compared with the row each entry belongs to.
+__global__ __launch_bounds__(kBlockThreads) void k_entry_rows(
+    const int32_t *__restrict__ pos, int32_t M, int32_t nnz, int32_t T, int32_t ntiles,
+    const int32_t *__restrict__ tile_row, int32_t *__restrict__ rows) {
+  __shared__ int32_t sm[kWavesPerBlock][2 * kPrChunk + kPosSlack];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int32_t t = (int32_t)blockIdx.x * kWavesPerBlock + wave;
+  if (t >= ntiles) return;  // uniform over the wave
+  int32_t *s_row = sm[wave];
+  int32_t *s_pos = sm[wave] + kPrChunk;
+  constexpr int32_t PW = kPrChunk + kPosSlack;
+  const int32_t s = t * T;
+  const int32_t e = (int32_t)min((int64_t)nnz, (int64_t)s + T);
+  const int32_t r_hi = min(M - 1, tile_row[t + 1]);
+  int32_t r = tile_row[t];
+  for (int32_t cs = s; cs < e; cs += kPrChunk) {
+    const int32_t cn = min(kPrChunk, e - cs);
+    const int32_t rbase = r;
+    const int32_t pw = min(PW, M + 1 - rbase);
+    for (int32_t i = lane; i < pw; i += 64) s_pos[i] = pos[rbase + i];
+    wave_sync();
+    chunk_entry_rows(pos, s_pos, pw, rbase, r_hi, cs, cn, lane, s_row);
+    wave_sync();
+    for (int32_t i = lane; i < cn; i += 64) rows[cs + i] = s_row[i];
+    r = s_row[cn - 1];
+    wave_sync();
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_fill_zero(float *C,
                                                              int32_t M,
                                                              int32_t N,
@@ -171,6 +204,16 @@ hipError_t launch_tile_rows_fix(int vec, const KArgs &a, hipStream_t s) {
     case 8: hipLaunchKernelGGL((k_tile_rows_fix<8>), g, b, 0, s, a.pos, a.M, a.tile, a.ntiles, rows, fix, a.C, a.N, a.ldc); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_entry_rows(const int32_t *pos, int32_t M, int32_t nnz, int32_t T,
+                             const int32_t *tile_row, int32_t *rows, hipStream_t s) {
+  const int64_t ntiles = ((int64_t)nnz + T - 1) / T;
+  if (ntiles <= 0) return hipSuccess;
+  const int64_t grid = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL(k_entry_rows, dim3((unsigned)grid), dim3(kBlockThreads), 0, s, pos, M,
+                     nnz, T, (int32_t)ntiles, tile_row, rows);
   return hipGetLastError();
 }
 

@@ -73,6 +73,10 @@ hipError_t launch_tile_rows(const int32_t *pos, int32_t M, int32_t T,
 // the empty rows of a.C (vec floats per store), so that the fix-up only sums
 // the carries of rows that span tiles.
 hipError_t launch_tile_rows_fix(int vec, const KArgs &a, hipStream_t s);
+// rows[p] for every stored entry p: its row as eb_pr's walk over tiles of T
+// entries derives it (tile_row as from launch_tile_rows).
+hipError_t launch_entry_rows(const int32_t *pos, int32_t M, int32_t nnz, int32_t T,
+                             const int32_t *tile_row, int32_t *rows, hipStream_t s);
 // Dynamic LDS bytes a launch of k_eb_sr / k_eb_pr uses.
 size_t eb_sr_lds_bytes(const KArgs &a);
 size_t eb_pr_lds_bytes(const KArgs &a);

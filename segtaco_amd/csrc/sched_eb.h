@@ -271,6 +271,30 @@ __device__ __forceinline__ void eb_sr_body(const KArgs &a, int32_t blk,
   }
 }
 
+// Row of every entry p = cs + i (i < cn) of a staged eb_pr chunk, into
+// s_row[i]: the last row j with pos[j] <= p (the non-empty row holding p),
+// by bisection in the LDS window s_pos = pos[rbase .. rbase + pw); an entry
+// beyond the window's last row continues with a bisection of pos in memory
+// up to r_hi, the last row the tile can touch. One wave; lane = lane id.
+__device__ __forceinline__ void chunk_entry_rows(const int32_t *__restrict__ pos,
+                                                 const int32_t *s_pos, int32_t pw,
+                                                 int32_t rbase, int32_t r_hi,
+                                                 int32_t cs, int32_t cn, int lane,
+                                                 int32_t *s_row) {
+  for (int32_t i = lane; i < cn; i += 64) {
+    const int32_t p = cs + i;
+    int32_t lo = 0, hi = pw - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (s_pos[mid] <= p)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    s_row[i] = (lo == pw - 1) ? row_of(pos, rbase + lo, r_hi, p) : rbase + lo;
+  }
+}
+
 // eb_pr over tile block `blk` (one tile per wave), column panel `panel`;
 // smem holds eb_pr_lds_bytes_fixed() bytes.
 template <int DT, int V, bool kWide, bool kOff32>
@@ -327,19 +351,7 @@ __device__ __forceinline__ void eb_pr_walk(const KArgs &a, int32_t blk,
     pw = min(PW, a.M + 1 - rbase);
     for (int32_t i = lane; i < pw; i += 64) s_pos[i] = pos[rbase + i];
     wave_sync();
-    // Row of every entry of the chunk: last j with pos[rbase + j] <= p.
-    for (int32_t i = lane; i < cn; i += 64) {
-      const int32_t p = cs + i;
-      int32_t lo = 0, hi = pw - 1;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (s_pos[mid] <= p)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      s_row[i] = (lo == pw - 1) ? row_of(pos, rbase + lo, r_hi, p) : rbase + lo;
-    }
+    chunk_entry_rows(pos, s_pos, pw, rbase, r_hi, cs, cn, lane, s_row);
     wave_sync();
     for (int32_t pb = 0; pb < cn; pb += P * kPrPasses) {
       int32_t rid[kPrPasses];

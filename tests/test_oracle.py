@@ -271,3 +271,138 @@ def test_half_rounding_matches_torch():
         tb = torch.from_numpy(x).to(tdt)
         assert np.array_equal(bits, tb.view(torch.int16).numpy().view(np.uint16))
         assert np.array_equal(back, tb.float().numpy())
+
+
+# ---------------------------------------------------------------------------
+# The error scale, per-entry rows and the tile partition the GPU checks use
+# ---------------------------------------------------------------------------
+
+def _dense_absbound(pos, crd, vals, B):
+    """|A| |B| in float64 from a dense copy of A."""
+    M = len(pos) - 1
+    A = np.zeros((M, B.shape[0]))
+    rows = np.repeat(np.arange(M), np.diff(pos))
+    A[rows, crd] = np.abs(vals.astype(np.float64)) if vals is not None else 1.0
+    return A @ np.abs(B.astype(np.float64))
+
+
+def _matches(S, ref):
+    return S.shape == ref.shape and np.allclose(S, ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_absbound_vs_dense_brute_force(seed):
+    rng = np.random.default_rng(400 + seed)
+    M, K, N = (int(x) for x in rng.integers(1, 70, 3))
+    pos, crd, vals, _ = _rand_csr(rng, M, K, 0.25)
+    empty = rng.random(M) < 0.3  # drop whole rows: empty rows must give 0
+    keep = ~np.repeat(empty, np.diff(pos))
+    crd, vals = crd[keep], vals[keep]
+    pos = np.concatenate([[0], np.cumsum(np.where(empty, 0, np.diff(pos)))]).astype(np.int32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    for v in (vals, None):
+        S = O.spmm_absbound(pos, crd, v, B)
+        ref = _dense_absbound(pos, crd, v, B)
+        assert _matches(S, ref)
+        assert np.all(S[np.diff(pos) == 0] == 0)
+        # planted mistakes in the scale that would loosen or break every
+        # tolerance stated on it: doubled, signs kept (|sum| instead of
+        # sum of |.|), one column of B shifted
+        assert not _matches(2.0 * S, ref)
+        signed = np.abs(O.spmm_f64(pos, crd, v, B))
+        if np.any(S > 0):
+            assert not _matches(signed, ref)
+        if N > 1 and np.any(S > 0):
+            assert not _matches(np.roll(S, 1, axis=1), ref) or np.allclose(S, S[:, :1])
+
+
+def test_absbound_planted_scale_loosens_metric():
+    """A scale twice too large halves every scaled error: an error just over
+    the 1e-5 bar passes under it. So the scale must be the exact one."""
+    rng = np.random.default_rng(17)
+    pos, crd, vals, _ = _rand_csr(rng, 40, 30, 0.3)
+    B = rng.uniform(-1, 1, (30, 4)).astype(np.float32)
+    ref = O.spmm_f64(pos, crd, vals, B)
+    S = O.spmm_absbound(pos, crd, vals, B)
+    i, k = np.unravel_index(np.argmax(S), S.shape)
+    C = ref.astype(np.float32)
+    C[i, k] = np.float32(ref[i, k] + 1.5e-5 * S[i, k])
+    assert O.scaled_err(C, ref, S) > 1e-5
+    assert O.scaled_err(C, ref, 2.0 * S) < 1e-5
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_row_of_nnz_brute_force(seed):
+    rng = np.random.default_rng(500 + seed)
+    M = int(rng.integers(1, 300))
+    pos = _random_pos(rng, M, 0.4)
+    rows = O.row_of_nnz(pos, M)
+    assert np.array_equal(rows, np.repeat(np.arange(M), np.diff(pos)))
+    for e in range(int(pos[M])):
+        assert rows[e] == O.segment_number(pos, M, e)
+
+
+def _tile_rows_brute(pos, T):
+    M = len(pos) - 1
+    rows = np.repeat(np.arange(M), np.diff(pos))
+    nt = (len(rows) + T - 1) // T
+    return np.concatenate([rows[np.arange(nt) * T], [M]]).astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tile_rows_brute_force(seed):
+    rng = np.random.default_rng(600 + seed)
+    M = int(rng.integers(1, 400))
+    pos = _random_pos(rng, M, 0.5)
+    pos_runs = pos.copy()
+    for T in (1, 2, 3, 8, 32, 256):
+        got = O.tile_rows(pos, M, T)
+        assert np.array_equal(got, _tile_rows_brute(pos, T)), T
+    # planted: the K21 search without the repair step lands inside runs of
+    # empty rows; with empty rows at tile starts it must differ
+    # (seven empty rows, then 5 entries: the search for entry 0 stops at row
+    # 4, the hand-traced answer above, and the repair walks on to row 7)
+    T = 4
+    pos_runs = np.array([0, 0, 0, 0, 0, 0, 0, 0, 5], np.int32)
+    no_repair = O.block_starts(pos_runs, 8, T)
+    assert list(no_repair) == [4, 7, 8]
+    assert list(O.tile_rows(pos_runs, 8, T)) == [7, 7, 8] == list(_tile_rows_brute(pos_runs, T))
+
+
+def _fix_t0_brute(pos, T):
+    """Tile by tile: the row holding the tile's first entry ends in this tile
+    after starting in an earlier one -> that earlier tile."""
+    M = len(pos) - 1
+    rows = np.repeat(np.arange(M), np.diff(pos))
+    nt = (len(rows) + T - 1) // T
+    out = np.full(nt, -1, np.int32)
+    for b in range(nt):
+        r = rows[b * T]
+        p0, p1 = int(pos[r]), int(pos[r + 1])
+        if p0 < b * T and p1 <= (b + 1) * T:
+            out[b] = p0 // T
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fix_t0_brute_force(seed):
+    rng = np.random.default_rng(700 + seed)
+    M = int(rng.integers(1, 300))
+    lens = rng.integers(0, 40, M)
+    lens[rng.random(M) < 0.3] = 0
+    pos = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    for T in (1, 3, 8, 16, 64):
+        got = O.fix_t0(pos, M, T)
+        ref = _fix_t0_brute(pos, T)
+        assert np.array_equal(got, ref), T
+    # planted: recording the start tile for every continued row (not only in
+    # the tile where it ends) differs whenever a row spans three tiles
+    lens = np.array([2, 20, 3], np.int64)
+    pos3 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ref = _fix_t0_brute(pos3, 4)
+    assert list(ref) == [-1, -1, -1, -1, -1, 0, 5]
+    rows = np.repeat(np.arange(3), lens)
+    planted = np.array([pos3[rows[b * 4]] // 4 if pos3[rows[b * 4]] < b * 4 else -1
+                        for b in range(len(ref))], np.int32)
+    assert not np.array_equal(planted, ref)
+    assert np.array_equal(O.fix_t0(pos3, 3, 4), ref)
