@@ -49,6 +49,17 @@ def compulsory_bytes(M, K_touched, N, nnz, s_v=4, s_b=4, s_c=4):
     return 4 * (M + 1) + nnz * (4 + s_v) + K_touched * N * s_b + M * N * s_c
 
 
+_TOUCHED = {}
+
+
+def k_touched(crd, K):
+    """Distinct columns referenced by crd (memoised per array)."""
+    key = (crd.ctypes.data, crd.size, K)
+    if key not in _TOUCHED:
+        _TOUCHED[key] = int(np.count_nonzero(np.bincount(crd, minlength=K)))
+    return _TOUCHED[key]
+
+
 def workloads():
     return {
         "powerlaw": dict(kind=S.GEN_POWERLAW, M=262144, K=262144, mean=16.0, sigma=1.5,
@@ -267,7 +278,8 @@ def run_leg(torch, ops, N, dtype, plans, steps, warmup, stream, seed_b):
     res = {}
     for name, plan in plans:
         rp, ms, per = time_plan(torch, A, Bd, S.DTYPES[dtype], C, N, plan, steps, warmup, stream)
-        res[name] = {"ms": ms, "ms_min": min(per), "gflops": flops / (ms * 1e-3) / 1e9,
+        res[name] = {"ms": ms, "ms_min": min(per), "ms_median": statistics.median(per),
+                     "gflops": flops / (ms * 1e-3) / 1e9,
                      "hbm_frac": cbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "plan": rp.as_dict()}
         log(f"N={N} {dtype} {name}: {ms*1e3:.1f} us/step  {res[name]['gflops']:.1f} GFLOP/s  "
             f"compulsory-BW frac {res[name]['hbm_frac']:.3f}")
@@ -276,10 +288,15 @@ def run_leg(torch, ops, N, dtype, plans, steps, warmup, stream, seed_b):
     return res, cbytes
 
 
-def best_form(res):
+def best_form(res, cbytes):
+    """The fastest schedule of a leg, with its compulsory bytes and the
+    fraction of HBM peak they make at its step time."""
     b = min(res, key=lambda k: res[k]["ms"])
-    return {"schedule": b, "ms": round(res[b]["ms"], 5), "gflops": round(res[b]["gflops"], 2),
-            "hbm_frac": round(res[b]["hbm_frac"], 4)}
+    return {"schedule": b, "ms": round(res[b]["ms"], 5),
+            "ms_median": round(res[b]["ms_median"], 5), "ms_min": round(res[b]["ms_min"], 5),
+            "gflops": round(res[b]["gflops"], 2), "compulsory_bytes": cbytes,
+            "hbm_frac": round(res[b]["hbm_frac"], 4),
+            "others": {k: round(v["ms"], 5) for k, v in res.items() if k != b}}
 
 
 def powerlaw_batch_set(n, seed=41):
@@ -369,9 +386,14 @@ def batch_leg(torch, mats, dtype, steps, warmup, stream):
     e1.record(stream)
     torch.cuda.synchronize()
     ms_seq = e0.elapsed_time(e1)
+    s_e = ELEM[dtype]
+    cbytes = sum(compulsory_bytes(M, k_touched(crd, M), N, int(pos[-1]), s_e, s_e)
+                 for pos, crd, _, M, N in mats)
     out = {"products": len(mats), "nnz": int(sum(int(m[0][-1]) for m in mats)),
            "units": batch.units(), "schedules": sched_hist,
            "ms": round(ms, 4), "gflops": round(flops / (ms * 1e-3) / 1e9, 2),
+           "compulsory_bytes": cbytes,
+           "hbm_frac": round(cbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "sequential_ms": round(ms_seq, 4),
            "sequential_gflops": round(flops / (ms_seq * 1e-3) / 1e9, 2)}
     log(f"batched {dtype}: {ms:.3f} ms/launch {out['gflops']} GFLOP/s "
@@ -427,6 +449,18 @@ def serve_leg(torch, mats, dtype, seconds, lam, headline):
     run(np.zeros(200), np.full(200, hid), np.full(200, 128, np.int32))
     head = server.stats()
     server.destroy()
+    # The same Poisson load with the CSR cache a third of the pool's bytes:
+    # misses upload matrices and evict the least recently used ones.
+    pool_bytes = sum(4 * (M + 1) + (4 + ELEM[dtype]) * int(pos[-1])
+                     for pos, _, _, M, _ in mats)
+    small = pool_bytes // 3
+    server = S.Server(cache_bytes=small, inflight_bytes=16 * GiB, max_batch=32, streams=2,
+                      dtype=dtype)
+    for pos, crd, vals, M, N in mats:
+        server.add_matrix(pos, crd, vals, M)
+    run(*trace(n, lam))
+    evicting = server.stats()
+    server.destroy()
     del bring, ring
     torch.cuda.empty_cache()
 
@@ -436,18 +470,73 @@ def serve_leg(torch, mats, dtype, seconds, lam, headline):
                          "max_us", "gflops", "cache_misses", "evictions",
                          "max_concurrent_ticks")}
     out = {"dtype": dtype, "lambda_per_s": lam, "requests": n,
+           "pool_csr_bytes": pool_bytes, "cache_bytes": 64 * GiB,
            "poisson": pick(poisson), "saturated": pick(burst),
            "powerlaw_n128_saturated": pick(head),
-           "latency_scope": "submit -> completion on the device (B, C device-resident)"}
+           "poisson_cache_third_of_pool": dict(pick(evicting), cache_bytes=small),
+           "latency_scope": "submit -> completion on the device (B, C device-resident; "
+                            "uploads of missed matrices included)"}
     log(f"served {dtype}: p50 {poisson['p50_us']:.0f} us p99 {poisson['p99_us']:.0f} us at "
         f"{lam}/s; saturated {burst['gflops']:.0f} GFLOP/s; power-law N=128 "
         f"{head['gflops']:.0f} GFLOP/s")
     return out
 
 
-def cpu_baseline(ops, N, flops, budget_s=10.0):
-    """The reference's CPU path (Eigen 3.4.0 via oracle/_ref) on the headline
-    matrix, all host threads (capped at 16), about budget_s of CPU work."""
+def host_cpu_info():
+    """This host's CPUs as the CPU baseline sees them: logical CPUs in the
+    system and in this process's affinity mask, the physical cores among the
+    latter (distinct (package, core) pairs), the model name, the cgroup CPU
+    quota if one is set, and OMP_NUM_THREADS as inherited."""
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in allowed:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            with open(f"{base}/physical_package_id") as f1, open(f"{base}/core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, p = f.read().split()
+            if q != "max":
+                quota = round(int(q) / int(p), 2)
+    except (OSError, ValueError):
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(allowed),
+            "physical_cores": len(cores), "model": model, "cgroup_cpu_quota": quota,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def ref_build_flags():
+    """The g++ flags oracle/Makefile builds the reference's Eigen path with."""
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as f:
+            for line in f:
+                if line.startswith("CXXFLAGS_REF"):
+                    return line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(ops, N, flops, budget_s=12.0):
+    """The reference's CPU path (Eigen 3.4.0 built from the reference's
+    sources, oracle/_ref) on the headline matrix: median of up to 25 steps
+    after one warm-up (fewer when a step is slow, so each run is about
+    budget_s), once on every physical core this process may use and once on
+    one thread. The all-core figure is the baseline `value`."""
     try:
         import oracle as O
     except Exception as e:  # pragma: no cover
@@ -456,19 +545,34 @@ def cpu_baseline(ops, N, flops, budget_s=10.0):
     if O.ref_lib() is None:
         return {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference",
                 "sample": "oracle/_ref not built"}
+    info = host_cpu_info()
     B = S.gen_dense(3, ops.K, N)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    t0 = time.time()
-    times, _ = O.ref_eigen_spmm_timed(ops.pos, ops.crd, ops.vals, B, threads, 1)
-    trials = int(max(3, min(500, budget_s / max(times[0], 1e-3))))
-    times, _ = O.ref_eigen_spmm_timed(ops.pos, ops.crd, ops.vals, B, threads, trials)
-    med = statistics.median(times)
-    return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
-            "kind": "reference",
+
+    def run(threads):
+        t0 = time.time()
+        times, _ = O.ref_eigen_spmm_timed(ops.pos, ops.crd, ops.vals, B, threads, 1)
+        trials = int(max(3, min(25, budget_s / max(times[0], 1e-3))))
+        times, _ = O.ref_eigen_spmm_timed(ops.pos, ops.crd, ops.vals, B, threads, trials)
+        med = statistics.median(times)
+        log(f"CPU baseline {threads} threads: {med*1e3:.1f} ms/step ({time.time()-t0:.1f} s)")
+        return {"gflops": round(flops / med / 1e9, 3), "ms_median": round(med * 1e3, 3),
+                "ms_min": round(min(times) * 1e3, 3), "trials": trials, "threads": threads}
+
+    # every physical core of the CPU set, unless a cgroup quota grants fewer
+    threads = info["physical_cores"]
+    if info["cgroup_cpu_quota"]:
+        threads = min(threads, int(np.ceil(info["cgroup_cpu_quota"])))
+    all_cores = run(max(1, threads))
+    one = run(1)
+    flags = ref_build_flags()
+    return {"value": all_cores["gflops"], "unit": "GFLOP/s", "cores": all_cores["threads"],
+            "kind": "reference", "single_thread": one, "all_cores": all_cores, "host": info,
+            "build": f"g++ {flags}" if flags else None,
             "sample": (f"headline matrix ({ops.M}^2, {ops.nnz} nnz) x B[{ops.K},{N}] FP32, "
-                       f"Eigen 3.4.0 C.noalias()=A*B (g++ -O3 -fopenmp), {threads} threads, "
-                       f"median of {trials} steps after one warm-up ({med*1e3:.1f} ms/step; "
-                       f"{time.time()-t0:.1f} s)")}
+                       f"Eigen 3.4.0 C.noalias()=A*B from the reference's ds.h types, "
+                       f"{all_cores['threads']} threads (the physical cores of this process's "
+                       f"CPU set, within its cgroup quota) and 1 thread, median of "
+                       f"{all_cores['trials']} / {one['trials']} steps after one warm-up")}
 
 
 def main():
@@ -545,6 +649,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 5),
+        "ms_per_step_median": round(r["ms_median"], 5),
+        "ms_per_step_min": round(r["ms_min"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -569,7 +675,8 @@ def main():
                               "C); traffic = rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 per launch "
                               "from the committed profile of this kernel; traffic_frac = traffic / "
                               "main-kernel time / peak")},
-        "schedules": {k: {"ms": round(v["ms"], 5), "gflops": round(v["gflops"], 2),
+        "schedules": {k: {"ms": round(v["ms"], 5), "ms_median": round(v["ms_median"], 5),
+                          "ms_min": round(v["ms_min"], 5), "gflops": round(v["gflops"], 2),
                           "hbm_frac": round(v["hbm_frac"], 4)} for k, v in head.items()},
     }
 
@@ -587,18 +694,18 @@ def main():
     if "forms" in legs:
         forms = {}
         for dtype in ("f32", "bf16", "fp16"):
-            res, _ = run_leg(torch, pl, 256, dtype, [("eb_sr", S.make_plan("eb_sr")),
-                                                    ("rb_pr", S.make_plan("rb_pr"))],
-                             args.steps, args.warmup, stream, 4)
-            forms[f"powerlaw_n256_{dtype}"] = best_form(res)
+            res, cb = run_leg(torch, pl, 256, dtype, [("eb_sr", S.make_plan("eb_sr")),
+                                                     ("rb_pr", S.make_plan("rb_pr"))],
+                              args.steps, args.warmup, stream, 4)
+            forms[f"powerlaw_n256_{dtype}"] = best_form(res, cb)
         pl._vals.clear()
         torch.cuda.empty_cache()
         banded = Operands(torch, W["banded"])
-        res, _ = run_leg(torch, banded, 256, "bf16", [("eb_sr", S.make_plan("eb_sr")),
-                                                     ("rb_sr", S.make_plan("rb_sr")),
-                                                     ("bd", S.make_plan("bd"))],
-                         max(5, args.steps // 5), 2, stream, 10)
-        forms["banded_1048576_n256_bf16"] = best_form(res)
+        res, cb = run_leg(torch, banded, 256, "bf16", [("eb_sr", S.make_plan("eb_sr")),
+                                                      ("rb_sr", S.make_plan("rb_sr")),
+                                                      ("bd", S.make_plan("bd"))],
+                          max(5, args.steps // 5), 2, stream, 10)
+        forms["banded_1048576_n256_bf16"] = best_form(res, cb)
         del banded
         torch.cuda.empty_cache()
         f32 = forms["powerlaw_n256_f32"]["gflops"]

@@ -62,7 +62,11 @@ int32_t segtaco_server_add_matrix(segtaco_server_t s, int32_t M, int32_t K,
                                   int32_t *matrix_id);
 /* Queues C = A[matrix_id] * B (device pointers). *request_id identifies it.
  * A request that can never fit the configured capacities completes at once
- * with status SEGTACO_ERR_CAPACITY. */
+ * with status SEGTACO_ERR_CAPACITY, and so does a request at the head of
+ * the queue that cannot be admitted while nothing runs (what it needs is
+ * held by nothing the server can release). After a HIP error in a tick the
+ * tick's requests complete with SEGTACO_ERR_HIP, so do all requests still
+ * queued, and further submissions are refused. */
 int32_t segtaco_server_submit(segtaco_server_t s, int32_t matrix_id,
                               const void *B, int64_t ldb, int32_t N, float *C,
                               int64_t ldc, int64_t *request_id);
@@ -86,6 +90,10 @@ int32_t segtaco_server_reset_stats(segtaco_server_t s);
 /* Plan the server uses for (matrix, N). */
 int32_t segtaco_server_plan(segtaco_server_t s, int32_t matrix_id, int32_t N,
                             segtaco_plan_t *out);
+/* Error-path testing: the launch of the tick `ticks_ahead` ticks from now
+ * (1 = the next one) reports a HIP error without running any of its device
+ * work, exactly as a failed launch would be handled. */
+int32_t segtaco_server_debug_fail_tick(segtaco_server_t s, int64_t ticks_ahead);
 int32_t segtaco_server_destroy(segtaco_server_t s);
 
 #ifdef __cplusplus

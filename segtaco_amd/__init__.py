@@ -33,6 +33,7 @@ SCHEDULES = {"rb_sr": RB_SR, "rb_pr": RB_PR, "eb_sr": EB_SR, "eb_pr": EB_PR, "eb
 DT_F32, DT_BF16, DT_F16 = 0, 1, 2
 DTYPES = {"f32": DT_F32, "bf16": DT_BF16, "fp16": DT_F16}
 OK = 0
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
 ERRMODE_EXIT, ERRMODE_RECORD = 0, 1
 GEN_UNIFORM, GEN_POWERLAW, GEN_BANDED = 0, 1, 2
 
@@ -287,6 +288,8 @@ def lib() -> ctypes.CDLL:
     L.segtaco_server_plan.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Plan)]
     L.segtaco_server_destroy.restype = ctypes.c_int32
     L.segtaco_server_destroy.argtypes = [_vp]
+    L.segtaco_server_debug_fail_tick.restype = ctypes.c_int32
+    L.segtaco_server_debug_fail_tick.argtypes = [_vp, ctypes.c_int64]
     _i32pp = ctypes.POINTER(_i32p)
     L.segtaco_mtx_read.restype = ctypes.c_int32
     L.segtaco_mtx_read.argtypes = [ctypes.c_char_p, ctypes.c_int32, _i32p, _i32p, _i32p, _i32pp,
@@ -881,6 +884,11 @@ class Server:
 
     def reset_stats(self) -> None:
         check(lib().segtaco_server_reset_stats(self.handle))
+
+    def debug_fail_tick(self, ticks_ahead: int = 1) -> None:
+        """Error-path testing: that tick's launch reports a HIP error without
+        running."""
+        check(lib().segtaco_server_debug_fail_tick(self.handle, ticks_ahead))
 
     def plan(self, mid: int, N: int) -> Plan:
         p = Plan()

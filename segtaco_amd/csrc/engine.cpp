@@ -533,6 +533,12 @@ int32_t plan_resolve_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
   return resolve(A, N, b_dtype, ldb, ldc, nullptr, nullptr, plan);
 }
 
+int32_t plan_resolve_ptrs_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
+                               int64_t ldb, int64_t ldc, const void *B, const void *C,
+                               segtaco_plan_t *plan) {
+  return resolve(A, N, b_dtype, ldb, ldc, B, C, plan);
+}
+
 }  // namespace segtaco
 
 // ---------------------------------------------------------------------------

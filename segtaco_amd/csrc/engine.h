@@ -60,6 +60,23 @@ constexpr int32_t kMaxNnz = 0x7fffffff - (1 << 23);
 // Compute units of the current device (256 on MI355X; queried once).
 int32_t device_cus();
 
+// The device API's implementation (engine.cpp). ev: 4 optional events
+// recorded around partition, main kernel and fix-up.
+int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                  int64_t ldb, int32_t N, float *C, int64_t ldc,
+                  const segtaco_plan_t *plan, void *ws, size_t ws_bytes,
+                  hipStream_t stream, hipEvent_t *ev);
+size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
+                                 const segtaco_plan_t *plan);
+// Resolves open plan fields without operand pointers (ld's as given).
+int32_t plan_resolve_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
+                          int64_t ldb, int64_t ldc, segtaco_plan_t *plan);
+// The same with the operand pointers, whose alignment bounds the vector
+// width: the plan segtaco_spmm runs for these operands.
+int32_t plan_resolve_ptrs_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
+                               int64_t ldb, int64_t ldc, const void *B, const void *C,
+                               segtaco_plan_t *plan);
+
 // Default element-balanced tile of a product with nnz entries: `tile`,
 // halved (not below 8) while the product would have fewer than two
 // workgroups of `tiles_per_block` tiles per compute unit. Each lane group

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <numeric>
 #include <string>
@@ -180,8 +181,18 @@ int32_t segtaco_mtx_read(const char *path, int32_t flags, int32_t *nrow,
                      m, n, e);
   info.entries = (int32_t)e;
   const int values = info.field == 2 ? 0 : (info.field == 1 ? 2 : 1);
+  // The header's entry count is not trusted for the reservation: every entry
+  // takes at least 4 bytes of text ("1 1\n"), so the bytes left bound it.
+  // The vector may still grow past what the host can give: that is a status,
+  // not an exception escaping the C ABI.
   std::vector<uint64_t> keys;
-  keys.reserve((size_t)(info.symmetry == 1 ? 2 * e : e));
+  const size_t left_entries = (size_t)(sc.end - sc.p) / 4 + 1;
+  const size_t want = (size_t)e < left_entries ? (size_t)e : left_entries;
+  try {
+    keys.reserve(info.symmetry == 1 ? 2 * want : want);
+  } catch (const std::exception &) {
+    return set_error(SEGTACO_ERR_NOMEM, "mtx: out of host memory");
+  }
   for (long long i = 0; i < e; ++i) {
     int64_t r, c;
     if (!sc.integer(&r) || !sc.integer(&c))
@@ -195,14 +206,18 @@ int32_t segtaco_mtx_read(const char *path, int32_t flags, int32_t *nrow,
     if (r < 1 || r > m || c < 1 || c > n)
       return set_error(SEGTACO_ERR_INVALID, "mtx: entry %lld (%lld, %lld) outside %lld x %lld",
                        i + 1, (long long)r, (long long)c, m, n);
-    keys.push_back(key_of((int32_t)(r - 1), (int32_t)(c - 1)));
-    if (info.symmetry == 1 && r != c) {
-      // the mirrored entry must fit too (a non-square "symmetric" file)
-      if (c > m || r > n)
-        return set_error(SEGTACO_ERR_INVALID,
-                         "mtx: entry %lld (%lld, %lld) mirrors outside %lld x %lld", i + 1,
-                         (long long)r, (long long)c, m, n);
-      keys.push_back(key_of((int32_t)(c - 1), (int32_t)(r - 1)));
+    try {
+      keys.push_back(key_of((int32_t)(r - 1), (int32_t)(c - 1)));
+      if (info.symmetry == 1 && r != c) {
+        // the mirrored entry must fit too (a non-square "symmetric" file)
+        if (c > m || r > n)
+          return set_error(SEGTACO_ERR_INVALID,
+                           "mtx: entry %lld (%lld, %lld) mirrors outside %lld x %lld", i + 1,
+                           (long long)r, (long long)c, m, n);
+        keys.push_back(key_of((int32_t)(c - 1), (int32_t)(r - 1)));
+      }
+    } catch (const std::exception &) {
+      return set_error(SEGTACO_ERR_NOMEM, "mtx: out of host memory");
     }
   }
   std::sort(keys.begin(), keys.end());

@@ -152,7 +152,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
   __shared__ int32_t s_win[2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   constexpr int32_t kCols = BChunk<CT>::kCols;  // panel width
-  const int32_t rb = blockIdx.x / a.panels, panel = blockIdx.x % a.panels;
+  // XCD-contiguous order: workgroup b runs on XCD b % 8, and XCD x takes the
+  // units x*per .. x*per + per - 1 in ascending order, so the blocks resident
+  // on one XCD at a time are neighbours whose B windows overlap (a banded
+  // block's 96-row window shares 64 rows with each neighbour) and its L2
+  // serves the overlap. Round-robin order put neighbours on different XCDs,
+  // and every block fetched its whole window from memory.
+  const int32_t units = a.nblocks * a.panels;
+  const int32_t per = (units + 7) / 8;
+  const int32_t u = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  if (u >= units) return;  // padding of the last round (whole workgroup)
+  const int32_t rb = u / a.panels, panel = u % a.panels;
   const int32_t r0 = rb * kBdRows;
   const int32_t rows = min(kBdRows, a.M - r0);
   const int32_t col0 = panel * kCols;
@@ -195,6 +205,29 @@ __global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
   BChunk<CT> ch;
   bd_load_chunk<CT>(a, cmin, W, 0, col0, vec8, ch);
   const int32_t pb = s_pos[0], n = s_pos[rows] - pb;
+  // The window above and the scatter below rely on every row's columns
+  // being strictly increasing: then a row's first and last entries bound it
+  // and no two entries share a tile cell. A block where an entry falls
+  // outside the window, or a column does not exceed its predecessor other
+  // than where a row starts (unsorted or duplicate columns), takes the
+  // per-row walk, which sums every entry in CSR order.
+  {
+    int bad = 0;
+    for (int32_t j = tid; j < n; j += kBlockThreads) {
+      const int32_t p = pb + j;
+      const int32_t c = a.crd[p];
+      bad |= (c < cmin) | (c > cmax);
+      if (j > 0 && a.crd[p - 1] >= c) {
+        bool starts = false;  // rare: where a row starts (or a violation)
+        for (int32_t i = 1; i < rows; ++i) starts |= s_pos[i] == p;
+        bad |= !starts;
+      }
+    }
+    if (__syncthreads_or(bad)) {
+      bd_rows_valu<DT>(a, r0, rows, col0, kCols);
+      return;
+    }
+  }
   constexpr int kScat = 8;  // entries per thread per round
   for (int32_t i = wave; i < kBdRows; i += kWavesPerBlock)
     for (int32_t w = lane; w < Wk / 2; w += 64)
@@ -271,8 +304,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
 int32_t bd_col_tiles(int32_t N) { return N > kWavesPerBlock * 32 ? 2 : 1; }
 
 hipError_t launch_bd(int dt, const KArgs &a, hipStream_t s) {
-  const int64_t grid = (int64_t)a.nblocks * a.panels;
-  if (grid <= 0) return hipSuccess;
+  const int64_t units = (int64_t)a.nblocks * a.panels;
+  if (units <= 0) return hipSuccess;
+  if (units > 0x7fffffff - 8) return hipErrorInvalidConfiguration;
+  const int64_t grid = 8 * ((units + 7) / 8);  // the kernel's XCD-contiguous decode
   if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(kBlockThreads);
   const int ct = bd_col_tiles(a.N);

@@ -157,6 +157,9 @@ struct Slot {
   std::vector<int64_t> reqs;
   int64_t tick_id = -1;
   bool busy = false;
+  // SEGTACO_OK, or SEGTACO_ERR_HIP when a step of the running tick's launch
+  // or its completion query failed: its requests then retire with that error
+  int32_t status = SEGTACO_OK;
 };
 
 size_t csr_bytes(const MatrixRec &m) {
@@ -202,6 +205,9 @@ struct segtaco_server {
   Clock::time_point first_submit, last_done;
   int32_t max_conc = 0;
   int32_t fatal = SEGTACO_OK;
+  // error-path testing: the launch of this tick (1-based) reports a HIP
+  // error without running (segtaco_server_debug_fail_tick); 0 = never
+  int64_t fail_tick = 0;
 };
 
 namespace {
@@ -231,9 +237,12 @@ void evict(segtaco_server *s, MatrixRec &m) {
 }
 
 // Allocates `bytes` in the CSR cache, evicting least-recently-used matrices
-// that no tick uses and whose uploads have landed; `keep` is never evicted.
-bool cache_alloc(segtaco_server *s, size_t bytes, const MatrixRec *keep,
-                 size_t *off) {
+// that no tick uses and whose uploads have landed. `keep` (the matrix being
+// admitted) is never evicted; its tile-row arrays for other tile sizes are
+// released, oldest first, once no other matrix can go, provided only the
+// request being admitted pins it (no running tick and no earlier request of
+// this tick uses the matrix, so none of those arrays is in use).
+bool cache_alloc(segtaco_server *s, size_t bytes, MatrixRec *keep, size_t *off) {
   while (!s->cache.alloc(bytes, off)) {
     MatrixRec *victim = nullptr;
     for (auto &mp : s->mats) {
@@ -245,8 +254,17 @@ bool cache_alloc(segtaco_server *s, size_t bytes, const MatrixRec *keep,
       }
       if (!victim || m.last_use < victim->last_use) victim = &m;
     }
-    if (!victim) return false;
-    evict(s, *victim);
+    if (victim) {
+      evict(s, *victim);
+      continue;
+    }
+    if (keep && keep->inflight == 1 && !keep->tiles.empty()) {
+      const TileRows t = keep->tiles.front();
+      keep->tiles.erase(keep->tiles.begin());
+      s->cache.release(t.off, t.bytes);
+      continue;
+    }
+    return false;
   }
   return true;
 }
@@ -332,21 +350,40 @@ void finish_request(segtaco_server *s, int64_t id, int32_t status,
   }
 }
 
+// Retires the tick of `slot`: its requests finish with the tick's status. A
+// tick whose launch or completion query failed may still have device work
+// queued (its completion event was never recorded, or cannot be read), so
+// its stream is drained before the workspace and pins are released.
 void retire(segtaco_server *s, Slot &slot) {
+  if (slot.status != SEGTACO_OK) (void)hipStreamSynchronize(slot.stream);
   const auto now = Clock::now();
   for (int64_t id : slot.reqs) {
     Req &r = s->reqs[id];
     if (r.ws_bytes) s->inflight.release(r.ws_off, r.ws_bytes);
     s->mats[r.mat]->inflight--;
-    finish_request(s, id, SEGTACO_OK, now);
+    finish_request(s, id, slot.status, now);
   }
   slot.reqs.clear();
   slot.busy = false;
+  slot.status = SEGTACO_OK;
+  s->cv_done.notify_all();
+}
+
+// After a HIP error nothing more is launched: every request still waiting
+// for admission finishes with SEGTACO_ERR_HIP.
+void fail_pending(segtaco_server *s) {
+  const auto now = Clock::now();
+  while (!s->pending.empty()) {
+    const int64_t id = s->pending.front();
+    s->pending.pop_front();
+    finish_request(s, id, SEGTACO_ERR_HIP, now);
+  }
   s->cv_done.notify_all();
 }
 
 // Admits pending requests into slot `si` and launches them. Caller holds mu.
 void launch_tick(segtaco_server *s, int32_t si) {
+  if (s->fatal != SEGTACO_OK) return;
   Slot &slot = s->slots[si];
   const int64_t tick = s->ticks + 1;
   std::vector<int64_t> chosen;
@@ -438,10 +475,11 @@ void launch_tick(segtaco_server *s, int32_t si) {
   } else {
     grid = 1024;
   }
-  hipError_t e = hipSuccess;
+  // a forced failure (error-path testing) skips every launch of the tick
+  hipError_t e = tick == s->fail_tick ? hipErrorLaunchFailure : hipSuccess;
   // one copy: counter, both prefix arrays, the first n descriptors
   *slot.h_counter = 0ull;
-  if (n > 0)
+  if (e == hipSuccess && n > 0)
     e = hipMemcpyAsync(slot.d_pack, slot.h_pack,
                        reinterpret_cast<char *>(slot.h_desc + n) - slot.h_pack,
                        hipMemcpyHostToDevice, slot.stream);
@@ -452,6 +490,9 @@ void launch_tick(segtaco_server *s, int32_t si) {
     e = launch_batch_fixup(slot.d_desc, n, slot.d_fix, total_fix, slot.stream);
   if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);
   if (e != hipSuccess) s->fatal = SEGTACO_ERR_HIP;
+  // any failure while this tick was put together (an upload, a partition, a
+  // cross-stream wait or a launch) fails the tick's requests
+  slot.status = s->fatal != SEGTACO_OK ? SEGTACO_ERR_HIP : SEGTACO_OK;
   slot.reqs = chosen;
   slot.busy = true;
   slot.tick_id = tick;
@@ -485,24 +526,32 @@ void scheduler(segtaco_server *s) {
           progressed = true;
         } else if (q != hipErrorNotReady) {
           s->fatal = SEGTACO_ERR_HIP;
+          slot.status = SEGTACO_ERR_HIP;
           retire(s, slot);
+          progressed = true;
         }
       }
     }
+    if (s->fatal != SEGTACO_OK && !s->pending.empty()) fail_pending(s);
     bool any_busy = false;
     for (auto &slot : s->slots) any_busy = any_busy || slot.busy;
     if (s->stop && s->pending.empty() && !any_busy) break;
+    bool launched = false;
     if (!s->pending.empty()) {
       for (int32_t si = 0; si < (int32_t)s->slots.size(); ++si) {
         if (!s->slots[si].busy && !s->pending.empty()) {
           const int64_t before = s->ticks;
           launch_tick(s, si);
-          progressed = progressed || s->ticks != before;
+          launched = launched || s->ticks != before;
         }
       }
+      progressed = progressed || launched;
     }
-    if (s->stop && !any_busy && !s->pending.empty()) {
-      // Nothing runs and the head request cannot be admitted: give up on it.
+    if (!any_busy && !launched && !s->pending.empty()) {
+      // Nothing runs, so the whole in-flight arena is free and every cached
+      // matrix may be evicted, and still the head request cannot be
+      // admitted: it never will be. It fails with a capacity error and the
+      // requests behind it get their turn.
       const int64_t id = s->pending.front();
       s->pending.pop_front();
       finish_request(s, id, SEGTACO_ERR_CAPACITY, Clock::now());
@@ -740,6 +789,14 @@ int32_t segtaco_server_reset_stats(segtaco_server_t s) {
   s->flops_done = 0.0;
   s->have_first = false;
   s->max_conc = 0;
+  return SEGTACO_OK;
+}
+
+int32_t segtaco_server_debug_fail_tick(segtaco_server_t s, int64_t ticks_ahead) {
+  if (!s || ticks_ahead < 1)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_server_debug_fail_tick: bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->fail_tick = s->ticks + ticks_ahead;
   return SEGTACO_OK;
 }
 

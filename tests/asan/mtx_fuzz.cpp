@@ -89,6 +89,11 @@ int main(int argc, char **argv) {
   for (long it = 0; it < iters; ++it) {
     std::string text = valid_file(rng);
     if (rng() % 3) text = mutate(text, rng);
+    if (it % 50 == 7) {
+      // a short file whose header declares about 2^31 entries: rejected
+      // (entries missing) without reserving memory for the declared count
+      text = std::string(kBanners[rng() % 2 ? 2 : 1]) + "\n10 10 2147483000\n1 1 0.5\n2 1 1.5\n";
+    }
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f) return 3;
     std::fwrite(text.data(), 1, text.size(), f);

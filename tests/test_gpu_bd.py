@@ -110,6 +110,47 @@ def test_bd_parity(case, dtype, N):
     assert np.array_equal(C.view(np.uint32), C2.view(np.uint32)), "not deterministic"
 
 
+def _disorder(name):
+    """A banded matrix whose blocks break the sorted-unique column rule in
+    different ways: duplicate (row, column) entries (summed, as CSR stores
+    them), rows in descending column order, one row whose entries leave the
+    window its first and last entries suggest; blocks in between stay sorted
+    and take the matrix cores."""
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    pos, crd, vals = S.gen_csr(S.GEN_BANDED, 8, 32 * 12, 32 * 12, bw=24)
+    rows = [list(crd[pos[i]:pos[i + 1]]) for i in range(len(pos) - 1)]
+    rv = [list(vals[pos[i]:pos[i + 1]]) for i in range(len(pos) - 1)]
+    for i in range(32 * 1, 32 * 2, 3):  # block 1: duplicates
+        k = int(rng.integers(0, len(rows[i])))
+        rows[i].insert(k, rows[i][k])
+        rv[i].insert(k, float(rng.uniform(-1, 1)))
+    for i in range(32 * 4, 32 * 5):  # block 4: every row descending
+        rows[i], rv[i] = rows[i][::-1], rv[i][::-1]
+    i = 32 * 7 + 5  # block 7: one row's middle entry far outside the window
+    rows[i][len(rows[i]) // 2] = 32 * 12 - 1
+    i = 32 * 9 + 31  # block 9: the last row's first entry repeats as its second
+    rows[i].insert(1, rows[i][0])
+    rv[i].insert(1, 0.25)
+    pos = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    crd = np.array([c for r in rows for c in r], np.int32)
+    vals = np.array([v for r in rv for v in r], np.float32)
+    return pos, crd, vals, 32 * 12
+
+
+@pytest.mark.parametrize("N", [256, 64])
+@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+def test_bd_unsorted_and_duplicate_columns(dtype, N):
+    """bd checks each block's column order before scattering it: blocks with
+    duplicate, unsorted or out-of-window entries take the per-row walk and
+    still give the product of the stored entries (duplicates summed)."""
+    pos, crd, vals, K = _disorder("disorder")
+    B = np.random.default_rng(N + 1).uniform(-1, 1, (K, N)).astype(np.float32)
+    print(f"[bd] disorder {dtype} N={N} nnz={len(crd)}", flush=True)
+    C, C2 = _run(pos, crd, vals, B, dtype, runs=2)
+    _check(pos, crd, vals, B, C, dtype, f"disorder/{dtype}/N={N}")
+    assert np.array_equal(C.view(np.uint32), C2.view(np.uint32)), "not deterministic"
+
+
 @pytest.mark.parametrize("dtype", ["bf16", "fp16"])
 def test_bd_strided_unaligned_and_novals(dtype):
     """ldb = 261 (no 16-byte B loads), ldc = 300, and A without values."""

@@ -361,6 +361,37 @@ def test_primary_entry_points(name):
     assert ms > 0
 
 
+@pytest.mark.parametrize("name", ["spmm_rb_sr", "spmm_csr_gpu_taco", "spmm_eb_pr",
+                                  "spmm_rb_pr_256_2_16_8", "spmm_csr_gpu_cusparse"])
+def test_cxx_linkage_entry_points(name):
+    """The C++-linkage symbols — what an object compiled against the
+    reference's gpu_kernels.cuh binds to (the link itself is checked on the
+    CPU) — compute the product and set the timer."""
+    import ctypes
+    L = S.lib()
+    fn = getattr(L, f"_Z{len(name)}{name}P13taco_tensor_tS0_S0_")
+    fn.restype = None
+    fn.argtypes = [ctypes.POINTER(S.TacoTensor)] * 3
+    timer = L._Z20get_gpu_timer_resultv
+    timer.restype = ctypes.c_float
+    pos, crd, vals, B = _case("empty_runs_n20")
+    M, K = len(pos) - 1, B.shape[0]
+    C = np.full((M, B.shape[1]), np.nan, np.float32)
+    keep = []
+    tA = S._taco_csr(M, K, np.asarray(pos, np.int32), np.asarray(crd, np.int32),
+                     np.asarray(vals, np.float32), keep)
+    tB, tC = S._taco_dense(np.ascontiguousarray(B), keep), S._taco_dense(C, keep)
+    prev = L.segtaco_set_error_mode(S.ERRMODE_RECORD)
+    try:
+        L.segtaco_clear_error()
+        fn(ctypes.byref(tC), ctypes.byref(tA), ctypes.byref(tB))
+        S.check(L.segtaco_last_error())
+    finally:
+        L.segtaco_set_error_mode(prev)
+    _check(pos, crd, vals, B, C, what=f"C++ linkage {name}")
+    assert timer() > 0
+
+
 @pytest.mark.parametrize("N", [64, 37])
 def test_operator_surface_large_operands(N):
     """Operands above the staging threshold (crd, vals, B and C of several

@@ -199,3 +199,73 @@ def test_one_matrix_many_widths_across_streams():
         assert server.stats()["completed"] == len(reqs)
         _check_results(server, mats, reqs, dev, "f32")
         server.destroy()
+
+
+def test_failed_tick_reports_errors_and_stops():
+    """A tick whose launch fails (forced from the host: the launch reports a
+    HIP error and none of its device work runs) retires its requests with
+    SEGTACO_ERR_HIP instead of OK; requests still queued fail the same way,
+    later submissions are refused and drain reports the error. Requests of
+    ticks before the failure complete normally."""
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 31, 5000, 5000, mean=20)
+    server = S.Server(cache_bytes=64 * MiB, inflight_bytes=64 * MiB, max_batch=1, streams=1)
+    mid = server.add_matrix(pos, crd, vals, 5000)
+    B = dense_dev(S.gen_dense(32, 5000, 32))
+    Cs = [torch().full((5000, 32), float("nan"), dtype=torch().float32, device="cuda")
+          for _ in range(6)]
+    torch().cuda.synchronize()
+    ok_id = server.submit(mid, B.data_ptr(), 32, 32, Cs[0].data_ptr(), 32)
+    assert server.wait(ok_id, 30.0) == 0
+    server.debug_fail_tick(1)
+    ids = [server.submit(mid, B.data_ptr(), 32, 32, C.data_ptr(), 32) for C in Cs[1:]]
+    statuses = [server.wait(i, 30.0) for i in ids]
+    print(f"[serve fail] statuses {statuses}", flush=True)
+    assert statuses[0] == S.ERR_HIP
+    assert all(s == S.ERR_HIP for s in statuses)
+    with pytest.raises(S.SegtacoError) as ei:
+        server.submit(mid, B.data_ptr(), 32, 32, Cs[0].data_ptr(), 32)
+    assert ei.value.status == S.ERR_HIP
+    with pytest.raises(S.SegtacoError):
+        server.drain(10.0)
+    st = server.stats()
+    assert st["completed"] == 1 and st["rejected"] == 5
+    ref = O.spmm_f64(pos, crd, vals, S.gen_dense(32, 5000, 32))
+    err = O.scaled_err(Cs[0].cpu().numpy(), ref,
+                       O.spmm_absbound(pos, crd, vals, S.gen_dense(32, 5000, 32)))
+    assert err <= 1e-5
+    server.destroy()
+
+
+def test_small_cache_one_matrix_many_widths():
+    """A cache that holds the matrix and one tile-row array but not two: a
+    request at another width (another tile size) releases the matrix's idle
+    array instead of stalling, and every result is right."""
+    rng = np.random.default_rng(37)
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 41, 20000, 20000, mean=24)
+    M = nnz = None
+    M, nnz = 20000, int(pos[-1])
+    probe = S.Server(cache_bytes=256 * MiB, inflight_bytes=64 * MiB, max_batch=1, streams=1)
+    probe.add_matrix(pos, crd, vals, M)
+    Ns = [16, 128, 32, 64, 16, 128]
+    tiles = {N: probe.plan(0, N).tile for N in Ns}
+    probe.destroy()
+    assert len(set(tiles.values())) > 1, tiles
+    csr = -(-4 * (M + 1) // 256) * 256 + 2 * (-(-4 * nnz // 256) * 256)
+
+    def rows_bytes(T):
+        return 2 * (-(-4 * (-(-nnz // T) + 1) // 256) * 256)
+    cache = csr + max(rows_bytes(T) for T in tiles.values()) + 256
+    server = S.Server(cache_bytes=cache, inflight_bytes=256 * MiB, max_batch=1, streams=1)
+    server.add_matrix(pos, crd, vals, M)
+    mats = [(pos, crd, vals, M)]
+    reqs = [(0, N, rng.uniform(-1, 1, (M, N)).astype(np.float32)) for N in Ns]
+    dev = [(dense_dev(B), torch().full((M, N), float("nan"), dtype=torch().float32,
+                                       device="cuda")) for _, N, B in reqs]
+    torch().cuda.synchronize()
+    ids = []
+    for (mi, N, _), (Bd, C) in zip(reqs, dev):
+        ids.append(server.submit(mi, Bd.data_ptr(), N, N, C.data_ptr(), N))
+    statuses = [server.wait(i, 60.0) for i in ids]
+    assert statuses == [0] * len(reqs), statuses
+    _check_results(server, mats, reqs, dev, "f32")
+    server.destroy()
