@@ -248,6 +248,24 @@ int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                      const segtaco_plan_t *plan, void *workspace,
                      size_t workspace_bytes, void *stream);
 
+/* segtaco_spmm without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * The workspace is a fixed array of the library in device memory,
+ * SEGTACO_NOALLOC_WORKSPACE_BYTES long, one per GPU, held from the first call
+ * on that GPU until the process ends; the call itself allocates nothing.
+ * A product whose workspace (segtaco_spmm_workspace_bytes of its resolved
+ * plan) exceeds that size is refused with SEGTACO_ERR_CAPACITY before any
+ * device work. Results are identical to segtaco_spmm with the same plan.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams, or from different threads) race and must not be
+ * issued; calls in one stream's order are safe. The array keeps the last
+ * call's contents, and no call reads what an earlier one left there. */
+#define SEGTACO_NOALLOC_WORKSPACE_BYTES ((size_t)128 << 20)
+int32_t segtaco_spmm_noalloc(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                             int64_t ldb, int32_t N, float *C, int64_t ldc,
+                             const segtaco_plan_t *plan, void *stream);
+
 /* Runs segtaco_spmm `reps` times back to back on `stream` and returns the
  * mean ms of its three stages in out_ms[0..2]: row partition, main kernel,
  * carry fix-up (0 for stages a plan does not have), each timed with HIP

@@ -174,6 +174,10 @@ def lib() -> ctypes.CDLL:
     L.segtaco_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32, ctypes.c_int64,
                                ctypes.c_int32, _vp, ctypes.c_int64, ctypes.POINTER(Plan), _vp,
                                ctypes.c_size_t, _vp]
+    L.segtaco_spmm_noalloc.restype = ctypes.c_int32
+    L.segtaco_spmm_noalloc.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.c_int32, _vp, ctypes.c_int64,
+                                       ctypes.POINTER(Plan), _vp]
     L.segtaco_spmm_stage_times.restype = ctypes.c_int32
     L.segtaco_spmm_stage_times.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32,
                                            ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int64,
@@ -600,6 +604,17 @@ def spmm(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: in
     """C = A * B on device memory (asynchronous on `stream`)."""
     check(lib().segtaco_spmm(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
                              ctypes.byref(plan), ws_ptr or None, ws_bytes, stream or None))
+
+
+NOALLOC_WORKSPACE_BYTES = 128 << 20
+
+
+def spmm_noalloc(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: int,
+                 plan: Plan, stream: int | None = None) -> None:
+    """C = A * B with the library's fixed device workspace (no allocation;
+    calls overlapping on one GPU race)."""
+    check(lib().segtaco_spmm_noalloc(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
+                                     ctypes.byref(plan), stream or None))
 
 
 def spmm_stage_times(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: int,

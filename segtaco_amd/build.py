@@ -39,7 +39,8 @@ LIB_SOURCES = (
     [(s, os.path.splitext(s)[0], []) for s in (
         "k_rb.hip", "k_eb.hip", "k_batch.hip", "engine.cpp", "legacy.cpp", "legacy_cxx.cpp",
         "gen.cpp", "autotune.cpp", "batch.cpp", "serve.cpp", "io.cpp", "vendor.cpp",
-        "k_sddmm.hip", "sddmm.cpp", "k_cs.hip", "k_spmv.hip", "k_bd.hip", "staging.cpp")]
+        "k_sddmm.hip", "sddmm.cpp", "k_cs.hip", "k_spmv.hip", "k_bd.hip", "staging.cpp",
+        "noalloc.hip")]
     + [("k_batch_inst.hip", f"k_batch_{dt}_{sv}",
         [f"-DSEGTACO_BATCH_DT={dt}", f"-DSEGTACO_BATCH_SV={sv}"])
        for dt in range(3) for sv in range(2)]
