@@ -29,10 +29,13 @@
 //     banded uses MEAN as the half bandwidth; values U[-1, 1]);
 //   * --summary also appends one line per matrix with each row's minimum
 //     over the trials (resana.py:150-180's 61-column table) to
-//     <results_dir>/<experiment>_<N>_<hw>_summary.csv.
+//     <results_dir>/<experiment>_<N>_<hw>_summary.csv, and one with each
+//     row's mean over the trials rounded to 4 decimals (resana.py:28-33)
+//     to <results_dir>/<experiment>_<N>_<hw>_summary_mean.csv.
 // Only the C-ABI headers are included: no HIP header (reference: timers.h:90
 // pulls in cuda_runtime.h).
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -280,37 +283,49 @@ int main(int argc, char **argv) {
   std::cout << log_path + " has been opened" << std::endl;
   if (fresh) log << "kernel,platform,format,library,tensor,time\n";
   std::vector<float> best(kSweepRows, 0.0f);
+  std::vector<double> mean(kSweepRows, 0.0);
   for (int r = 0; r < kSweepRows; ++r) {
     const SweepRow &row = kSweep[r];
     float mn = 0.0f;
+    double sum = 0.0;
     for (int t = 0; t < o.trials; ++t) {
       row.fn(&op.tC, &op.tA, &op.tB);
       const float ms = get_gpu_timer_result();
       log << "spmm," << row.platform << "," << row.format << "," << row.library << ","
           << o.matrix_name << "," << ms << "\n";
       mn = t == 0 ? ms : std::min(mn, ms);
+      sum += ms;
     }
     best[r] = mn;
+    mean[r] = o.trials > 0 ? sum / o.trials : 0.0;
   }
   log.close();
   if (o.summary) {
-    const std::string sum_path = o.results_dir + "/" + o.experiment + "_" +
-                                 std::to_string(o.cols) + "_" + o.hardware + "_summary.csv";
-    const bool sfresh = file_is_empty(sum_path);
-    std::ofstream s(sum_path, std::ofstream::app);
-    if (sfresh) {
-      s << "dataset";
-      for (int r = 0; r < kSweepRows; ++r) {
-        if (!std::strcmp(kSweep[r].library, "alg2"))
-          s << ",alg2";
-        else
-          s << "," << kSweep[r].platform << kSweep[r].format << kSweep[r].library;
+    // resana.py's two tables: each row's minimum over the trials
+    // (resana.py:150-180) and its mean rounded to 4 decimals
+    // (resana.py:28-33), one line per matrix in each file
+    auto table = [&](const std::string &suffix, auto value) {
+      const std::string path = o.results_dir + "/" + o.experiment + "_" +
+                               std::to_string(o.cols) + "_" + o.hardware + suffix;
+      const bool sfresh = file_is_empty(path);
+      std::ofstream s(path, std::ofstream::app);
+      if (sfresh) {
+        s << "dataset";
+        for (int r = 0; r < kSweepRows; ++r) {
+          if (!std::strcmp(kSweep[r].library, "alg2"))
+            s << ",alg2";
+          else
+            s << "," << kSweep[r].platform << kSweep[r].format << kSweep[r].library;
+        }
+        s << "\n";
       }
+      s << o.matrix_name;
+      for (int r = 0; r < kSweepRows; ++r) s << "," << value(r);
       s << "\n";
-    }
-    s << o.matrix_name;
-    for (int r = 0; r < kSweepRows; ++r) s << "," << best[r];
-    s << "\n";
+    };
+    table("_summary.csv", [&](int r) { return best[r]; });
+    table("_summary_mean.csv",
+          [&](int r) { return std::round(mean[r] * 1e4) / 1e4; });
   }
   return 0;
 }

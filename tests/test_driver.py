@@ -95,6 +95,12 @@ def test_sweep_writes_reference_rows(tmp_path):
     assert len(summ[0]) == 62 and summ[1][0] == "head"
     mins = [min(float(body[3 * r + t][5]) for t in range(3)) for r in range(61)]
     assert np.allclose([float(x) for x in summ[1][1:]], mins, rtol=1e-5)
+    # resana's other table: the mean over the trials, rounded to 4 decimals
+    with open(tmp_path / "spmm_csr_gpu_8_mi355x_summary_mean.csv") as f:
+        summ_m = list(csv.reader(f))
+    assert summ_m[0] == summ[0] and summ_m[1][0] == "head"
+    means = [round(sum(float(body[3 * r + t][5]) for t in range(3)) / 3, 4) for r in range(61)]
+    assert np.allclose([float(x) for x in summ_m[1][1:]], means, atol=1.5e-4)
     # a second run appends without a second header
     r = _run([str(tmp_path), "head", "spmm_csr_gpu", str(tmp_path), "mi355x", "--trials", "1",
               "--cols", "8"])
