@@ -248,6 +248,32 @@ int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                      const segtaco_plan_t *plan, void *workspace,
                      size_t workspace_bytes, void *stream);
 
+/* Preparation of A for repeated products (A resident and unchanged between
+ * calls): writes into `workspace` what segtaco_spmm computes from A alone —
+ * for eb_cs the slot plan and the column-sorted records (the larger part of
+ * an eb_cs call), for eb_sr / eb_pr the tile rows — on `stream`, without
+ * reading B or C. The plan resolves as segtaco_plan_resolve resolves it for
+ * these ld's (no operand pointers). No counterpart in the reference, whose
+ * entry points redo this work on every call; the analogue of a vendor
+ * library's SpMM preprocessing stage. */
+int32_t segtaco_spmm_prepare(const segtaco_csr_t *A, int32_t b_dtype, int64_t ldb, int32_t N,
+                             int64_t ldc, const segtaco_plan_t *plan, void *workspace,
+                             size_t workspace_bytes, void *stream);
+
+/* segtaco_spmm on a workspace whose last preparation (by
+ * segtaco_spmm_prepare or by a segtaco_spmm call) was of this product: the
+ * same device arrays of A with unchanged contents, the same N and storage
+ * type, and a plan that resolves, for these operands, to the prepared one.
+ * Skips the preparation (eb_cs); every other step runs as in segtaco_spmm,
+ * and C is bit-identical to it. A workspace prepared for another product is
+ * refused (SEGTACO_ERR_INVALID) for eb_cs; whether A's contents changed is
+ * not checked. Calls on other streams than the preparation's must be ordered
+ * after it by the caller. */
+int32_t segtaco_spmm_prepared(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                              int64_t ldb, int32_t N, float *C, int64_t ldc,
+                              const segtaco_plan_t *plan, void *workspace,
+                              size_t workspace_bytes, void *stream);
+
 /* segtaco_spmm without a workspace argument, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). No counterpart in the reference.

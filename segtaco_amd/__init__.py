@@ -174,6 +174,12 @@ def lib() -> ctypes.CDLL:
     L.segtaco_spmm.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32, ctypes.c_int64,
                                ctypes.c_int32, _vp, ctypes.c_int64, ctypes.POINTER(Plan), _vp,
                                ctypes.c_size_t, _vp]
+    L.segtaco_spmm_prepare.restype = ctypes.c_int32
+    L.segtaco_spmm_prepare.argtypes = [ctypes.POINTER(CSR), ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Plan), _vp,
+                                       ctypes.c_size_t, _vp]
+    L.segtaco_spmm_prepared.restype = ctypes.c_int32
+    L.segtaco_spmm_prepared.argtypes = L.segtaco_spmm.argtypes
     L.segtaco_spmm_noalloc.restype = ctypes.c_int32
     L.segtaco_spmm_noalloc.argtypes = [ctypes.POINTER(CSR), _vp, ctypes.c_int32, ctypes.c_int64,
                                        ctypes.c_int32, _vp, ctypes.c_int64,
@@ -604,6 +610,23 @@ def spmm(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: in
     """C = A * B on device memory (asynchronous on `stream`)."""
     check(lib().segtaco_spmm(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
                              ctypes.byref(plan), ws_ptr or None, ws_bytes, stream or None))
+
+
+def spmm_prepare(A: CSR, b_dtype: int, ldb: int, N: int, ldc: int, plan: Plan,
+                 ws_ptr: int | None, ws_bytes: int, stream: int | None = None) -> None:
+    """Writes what a product with A computes from A alone (eb_cs: slot plan
+    and column-sorted records) into the workspace (asynchronous)."""
+    check(lib().segtaco_spmm_prepare(ctypes.byref(A), b_dtype, ldb, N, ldc, ctypes.byref(plan),
+                                     ws_ptr or None, ws_bytes, stream or None))
+
+
+def spmm_prepared(A: CSR, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int, ldc: int,
+                  plan: Plan, ws_ptr: int | None, ws_bytes: int,
+                  stream: int | None = None) -> None:
+    """spmm on a workspace prepared for this product (skips the preparation)."""
+    check(lib().segtaco_spmm_prepared(ctypes.byref(A), B_ptr, b_dtype, ldb, N, C_ptr, ldc,
+                                      ctypes.byref(plan), ws_ptr or None, ws_bytes,
+                                      stream or None))
 
 
 NOALLOC_WORKSPACE_BYTES = 128 << 20

@@ -9,13 +9,17 @@ namespace segtaco {
 
 constexpr int kCsThreads = 512;           // main kernel: 8 waves
 constexpr int kCsWaves = kCsThreads / 64;
-constexpr int kCsPrepThreads = 256;       // prep kernel: 4 waves
+constexpr int kCsPrepThreads = 256;       // prep kernels: one tile per wave
 constexpr int32_t kCsLanes = 8;           // lanes per entry (Q)
 constexpr int32_t kCsMaxGroup = kCsWaves * (64 / kCsLanes);  // tiles per unit
 constexpr int32_t kCsMaxTile = 512;       // entries per tile
-constexpr uint16_t kCsHead = 0xFFFF;      // slot code: row began before tile
-constexpr uint16_t kCsTail = 0xFFFE;      // slot code: row continues after
 constexpr size_t kCsLdsBudget = 152 * 1024;  // accumulator bytes per unit
+// One record per sorted entry, read with one load: {(column << kCsCodeBits)
+// | slot, value bits} in 64 bits when every column is below
+// 2^(32 - kCsCodeBits) (slots are below 2^kCsCodeBits), otherwise {column,
+// slot, value bits} in 96 bits. Value bits: the FP32 word, or the
+// 16-bit storage word zero-extended; 1.0 when A has no values.
+constexpr int kCsCodeBits = 12;
 
 struct CsArgs {
   int32_t M, K, N, nnz;
@@ -31,16 +35,24 @@ struct CsArgs {
   int32_t units, rounds;  // units = ceil(ntiles / group); units per block
   int32_t cap;          // LDS accumulator slots per unit
   int32_t elem;         // bytes per stored value
+  int32_t packed;       // 64-bit records (else 96)
+  uint32_t one_bits;    // 1.0 in the storage type (FP32, BF16 or FP16 bits)
   const int32_t *tile_row;
   float *carry_head, *carry_tail;
-  // written by k_cs_prep
-  int32_t *s_col;       // nnz: columns, each tile sorted
-  void *s_val;          // nnz: values in the same order (elem bytes)
-  uint16_t *s_slot;     // nnz: LDS slot, kCsHead or kCsTail
+  // written by the preparation kernels
+  void *s_rec;          // nnz records (column, code, value). Sweeping tiles:
+                        // sorted by (column, position), code = LDS slot
+                        // (2g / 2g + 1 = head / tail row of the unit's tile
+                        // g, from 2G on the unit's interior rows). Walking
+                        // tiles: CSR order, code 0 head, 1 tail, 2 + k the
+                        // tile's k-th interior row
   int32_t *slot_rows;   // units x cap: row of each interior slot
-  int32_t *n_slots;     // units
+  int32_t *walk_rows;   // ntiles x tile: row of a walking tile's k-th
+                        // interior row
+  int32_t *n_slots;     // units: slots used (2G + swept interior rows)
   int32_t *tile_count;  // ntiles: interior rows of each tile
-  uint8_t *tile_flags;  // ntiles: bit 0 head row, bit 1 tail row
+  uint8_t *tile_flags;  // ntiles: bit 0 head row, bit 1 tail row, bit 2
+                        // the tile walks
 };
 
 hipError_t launch_cs_prep(const CsArgs &a, hipStream_t s);

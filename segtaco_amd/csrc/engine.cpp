@@ -6,6 +6,8 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "bd.h"
@@ -49,32 +51,36 @@ EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile) {
 }
 
 // Workspace of an eb_cs plan: the eb layout (partition, tile rows, carries)
-// followed by the sorted copy of A and the unit tables k_cs_prep writes.
+// followed by the sorted copy of A (one record per entry: column, slot and
+// value, 64 bits when every column is below 2^20, else 96) and the unit
+// tables the preparation kernels write.
 struct CsLayout {
   EbLayout eb;
-  int32_t units, cap;
-  size_t col_off, val_off, slot_off, rows_off, nslot_off, count_off, flag_off, total;
+  int32_t units, cap, packed;
+  size_t rec_off, rows_off, walk_off, nslot_off, count_off, flag_off, total;
 };
 
 size_t device_max_lds();
 
-CsLayout cs_layout(int32_t nnz, int32_t N, int32_t tile, int32_t group,
-                   int32_t vec, int32_t elem, bool has_vals) {
+CsLayout cs_layout(int32_t nnz, int32_t K, int32_t N, int32_t tile, int32_t group,
+                   int32_t vec) {
   CsLayout L{};
   L.eb = eb_layout(nnz, N, tile);
   L.units = (int32_t)ceil_div(L.eb.ntiles, group);
   const size_t budget = device_max_lds() < kCsLdsBudget ? device_max_lds() : kCsLdsBudget;
-  L.cap = (int32_t)(budget / ((size_t)kCsLanes * vec * sizeof(float)));
+  const size_t cap = budget / ((size_t)kCsLanes * vec * sizeof(float));
+  const size_t max_code = (size_t)1 << kCsCodeBits;  // slots fit the code field
+  L.cap = (int32_t)(cap < max_code ? cap : max_code);
+  L.packed = K <= (1 << (32 - kCsCodeBits)) ? 1 : 0;
   size_t off = L.eb.total;
   auto take = [&](size_t bytes) {
     const size_t o = off;
     off = align_up(off + bytes, 256);
     return o;
   };
-  L.col_off = take(sizeof(int32_t) * (size_t)nnz);
-  L.val_off = take(has_vals ? (size_t)elem * nnz : 0);
-  L.slot_off = take(sizeof(uint16_t) * (size_t)nnz);
+  L.rec_off = take((L.packed ? 8 : 12) * (size_t)nnz);
   L.rows_off = take(sizeof(int32_t) * (size_t)L.units * L.cap);
+  L.walk_off = take(sizeof(int32_t) * (size_t)L.eb.ntiles * tile);
   L.nslot_off = take(sizeof(int32_t) * (size_t)L.units);
   L.count_off = take(sizeof(int32_t) * (size_t)L.eb.ntiles);
   L.flag_off = take((size_t)L.eb.ntiles);
@@ -300,9 +306,7 @@ namespace {
 size_t resolved_workspace_bytes(const segtaco_csr_t *A, int32_t N,
                                 const segtaco_plan_t &p, int32_t b_dtype) {
   if (p.schedule == SEGTACO_EB_CS)
-    return cs_layout(A->nnz, N, p.tile, p.group, p.vec, elem_bytes(b_dtype),
-                     A->vals != nullptr)
-        .total;
+    return cs_layout(A->nnz, A->K, N, p.tile, p.group, p.vec).total;
   if (p.schedule != SEGTACO_EB_SR && p.schedule != SEGTACO_EB_PR) return 0;
   return eb_layout(A->nnz, N, p.tile).total;
 }
@@ -357,27 +361,99 @@ int32_t spmm_workspace_layout_impl(const segtaco_csr_t *A, int32_t N,
   return SEGTACO_OK;
 }
 
+namespace {
+
+// What an eb_cs workspace holds: the product whose preparation (slot plan
+// and sorted records) was last written there. Every call that prepares
+// into a workspace records it (or forgets the workspace, for schedules whose
+// run repeats the preparation anyway); a run-only call checks it, so that a
+// workspace prepared for another matrix, width or plan is refused instead of
+// read. The check compares device pointers and sizes only: that A's arrays
+// still hold the same entries is the caller's promise.
+struct PrepKey {
+  const void *pos, *crd, *vals;
+  int32_t M, K, nnz, b_dtype, N;
+  segtaco_plan_t plan;
+  bool operator==(const PrepKey &o) const {
+    return pos == o.pos && crd == o.crd && vals == o.vals && M == o.M && K == o.K &&
+           nnz == o.nnz && b_dtype == o.b_dtype && N == o.N &&
+           std::memcmp(&plan, &o.plan, sizeof(plan)) == 0;
+  }
+};
+
+std::mutex g_prep_mu;
+std::unordered_map<const void *, PrepKey> g_prepared;
+constexpr size_t kMaxPrepared = 4096;  // forgotten beyond: run-only calls then fail
+
+PrepKey prep_key(const segtaco_csr_t *A, int32_t b_dtype, int32_t N,
+                 const segtaco_plan_t &plan) {
+  PrepKey k;
+  std::memset(&k, 0, sizeof(k));
+  k.pos = A->pos;
+  k.crd = A->crd;
+  k.vals = A->vals;
+  k.M = A->M;
+  k.K = A->K;
+  k.nnz = A->nnz;
+  k.b_dtype = b_dtype;
+  k.N = N;
+  k.plan = plan;
+  return k;
+}
+
+void prep_record(const void *ws, const PrepKey *k) {
+  if (!ws) return;
+  std::lock_guard<std::mutex> g(g_prep_mu);
+  if (!k) {
+    g_prepared.erase(ws);
+    return;
+  }
+  if (g_prepared.size() >= kMaxPrepared && !g_prepared.count(ws)) g_prepared.clear();
+  g_prepared[ws] = *k;
+}
+
+bool prep_matches(const void *ws, const PrepKey &k) {
+  std::lock_guard<std::mutex> g(g_prep_mu);
+  const auto it = g_prepared.find(ws);
+  return it != g_prepared.end() && it->second == k;
+}
+
+}  // namespace
+
 // ev (optional, 4 events): recorded before the partition, after it, after
 // the main kernel and after the fix-up (row-balanced plans have neither
 // partition nor fix-up; their events bracket nothing).
 int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                   int64_t ldb, int32_t N, float *C, int64_t ldc,
                   const segtaco_plan_t *plan_in, void *ws, size_t ws_bytes,
-                  hipStream_t stream, hipEvent_t *ev) {
+                  hipStream_t stream, hipEvent_t *ev, int32_t phases) {
   SEGTACO_RET(check_csr(A));
   if (!plan_in) return set_error(SEGTACO_ERR_INVALID, "null plan");
+  if (phases < kPhasePrepare || phases > kPhaseAll)
+    return set_error(SEGTACO_ERR_INVALID, "phases %d", phases);
+  const bool prepare = (phases & kPhasePrepare) != 0;
+  const bool run = (phases & kPhaseRun) != 0;
   if (N < 0) return set_error(SEGTACO_ERR_INVALID, "N < 0");
   if (A->M == 0 || N == 0) return SEGTACO_OK;
-  if (!C) return set_error(SEGTACO_ERR_INVALID, "null C");
+  if (run && !C) return set_error(SEGTACO_ERR_INVALID, "null C");
   if (ldc < N) return set_error(SEGTACO_ERR_INVALID, "ldc < N");
   if (A->nnz == 0) {
-    SEGTACO_HIP(launch_fill_zero(C, A->M, N, ldc, stream));
+    if (run) SEGTACO_HIP(launch_fill_zero(C, A->M, N, ldc, stream));
     return SEGTACO_OK;
   }
-  if (!B) return set_error(SEGTACO_ERR_INVALID, "null B");
+  if (run && !B) return set_error(SEGTACO_ERR_INVALID, "null B");
   if (ldb < N) return set_error(SEGTACO_ERR_INVALID, "ldb < N");
   segtaco_plan_t plan = *plan_in;
-  SEGTACO_RET(resolve(A, N, b_dtype, ldb, ldc, B, C, &plan));
+  SEGTACO_RET(resolve(A, N, b_dtype, ldb, ldc, run ? B : nullptr, run ? C : nullptr, &plan));
+  if (!run) {
+    B = nullptr;
+    C = nullptr;
+  }
+  const PrepKey key = prep_key(A, b_dtype, N, plan);
+  if (!prepare && plan.schedule == SEGTACO_EB_CS && !prep_matches(ws, key))
+    return set_error(SEGTACO_ERR_INVALID,
+                     "segtaco_spmm_prepared: the workspace was not prepared for this "
+                     "product (matrix, width, storage type and resolved plan)");
 
   KArgs a{};
   a.M = A->M;
@@ -404,6 +480,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   };
   switch (plan.schedule) {
     case SEGTACO_RB_SR: {
+      if (!run) return SEGTACO_OK;  // nothing depends on A alone
       const int64_t rows_per_block =
           (int64_t)kWavesPerBlock * (64 / plan.col_lanes) * plan.group;
       a.nblocks = (int32_t)ceil_div(A->M, rows_per_block);
@@ -415,6 +492,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
       return SEGTACO_OK;
     }
     case SEGTACO_RB_PR: {
+      if (!run) return SEGTACO_OK;  // nothing depends on A alone
       const int64_t rows_per_wave = 64 / (plan.col_lanes * plan.group);
       a.nblocks = (int32_t)ceil_div(A->M, rows_per_wave * kWavesPerBlock);
       SEGTACO_HIP(mark(0));
@@ -425,6 +503,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
       return SEGTACO_OK;
     }
     case SEGTACO_BD: {
+      if (!run) return SEGTACO_OK;  // nothing depends on A alone
       a.nblocks = (int32_t)ceil_div(A->M, (int64_t)kBdRows);
       a.panels = (int32_t)ceil_div(N, (int64_t)bd_col_tiles(N) * kWavesPerBlock * 32);
       SEGTACO_HIP(mark(0));
@@ -438,8 +517,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
       break;
   }
   if (plan.schedule == SEGTACO_EB_CS) {
-    const CsLayout L = cs_layout(A->nnz, N, plan.tile, plan.group, V,
-                                 elem_bytes(b_dtype), A->vals != nullptr);
+    const CsLayout L = cs_layout(A->nnz, A->K, N, plan.tile, plan.group, V);
     if (!ws || ws_bytes < L.total)
       return set_error(SEGTACO_ERR_INVALID,
                        "workspace of %zu bytes needed, %zu given", L.total,
@@ -471,27 +549,36 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     c.rounds = (int32_t)ceil_div(L.units, cus);
     c.cap = L.cap;
     c.elem = elem_bytes(b_dtype);
+    c.packed = L.packed;
+    c.one_bits = b_dtype == SEGTACO_DT_F32 ? 0x3f800000u
+                 : b_dtype == SEGTACO_DT_BF16 ? 0x3f80u
+                                              : 0x3c00u;
     c.tile_row = a.tile_row;
     c.carry_head = a.carry_head;
     c.carry_tail = a.carry_tail;
-    c.s_col = reinterpret_cast<int32_t *>(w + L.col_off);
-    c.s_val = A->vals ? static_cast<void *>(w + L.val_off) : nullptr;
-    c.s_slot = reinterpret_cast<uint16_t *>(w + L.slot_off);
+    c.s_rec = static_cast<void *>(w + L.rec_off);
     c.slot_rows = reinterpret_cast<int32_t *>(w + L.rows_off);
+    c.walk_rows = reinterpret_cast<int32_t *>(w + L.walk_off);
     c.n_slots = reinterpret_cast<int32_t *>(w + L.nslot_off);
     c.tile_count = reinterpret_cast<int32_t *>(w + L.count_off);
     c.tile_flags = reinterpret_cast<uint8_t *>(w + L.flag_off);
     // the fix-up runs with its own natural lane count for N
     a.lq = ilog2(natural_lanes(N, V));
     a.panels = (int32_t)ceil_div(N, (int64_t)(1 << a.lq) * V);
+    // The tile rows are rewritten on every call: the same kernel zeroes C's
+    // empty rows (none when C is null, in a preparation alone).
+    if (prepare) prep_record(ws, nullptr);
     SEGTACO_HIP(mark(0));
     SEGTACO_HIP(launch_tile_rows_fix(V, a, stream));
-    SEGTACO_HIP(launch_cs_prep(c, stream));
+    if (prepare) SEGTACO_HIP(launch_cs_prep(c, stream));
     SEGTACO_HIP(mark(1));
-    SEGTACO_HIP(launch_eb_cs(b_dtype, V, c, stream));
-    SEGTACO_HIP(mark(2));
-    SEGTACO_HIP(launch_fixup(V, a, stream));
-    SEGTACO_HIP(mark(3));
+    if (run) {
+      SEGTACO_HIP(launch_eb_cs(b_dtype, V, c, stream));
+      SEGTACO_HIP(mark(2));
+      SEGTACO_HIP(launch_fixup(V, a, stream));
+      SEGTACO_HIP(mark(3));
+    }
+    if (prepare) prep_record(ws, &key);
     return SEGTACO_OK;
   }
   // Element-balanced: partition, tiles, carry fix-up.
@@ -507,9 +594,11 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
   a.carry_head = reinterpret_cast<float *>(w + L.head_off);
   a.carry_tail = reinterpret_cast<float *>(w + L.tail_off);
   a.tile_row = reinterpret_cast<int32_t *>(w + L.rows_off);
+  if (prepare) prep_record(ws, nullptr);
   SEGTACO_HIP(mark(0));
   SEGTACO_HIP(launch_tile_rows_fix(V, a, stream));
   SEGTACO_HIP(mark(1));
+  if (!run) return SEGTACO_OK;
   if (plan.schedule == SEGTACO_EB_SR) {
     const int64_t tiles_per_block =
         (int64_t)kWavesPerBlock * (64 / plan.col_lanes);
@@ -615,6 +704,21 @@ int32_t segtaco_spmm(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                      size_t workspace_bytes, void *stream) {
   return spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace,
                    workspace_bytes, static_cast<hipStream_t>(stream), nullptr);
+}
+
+int32_t segtaco_spmm_prepare(const segtaco_csr_t *A, int32_t b_dtype, int64_t ldb, int32_t N,
+                             int64_t ldc, const segtaco_plan_t *plan, void *workspace,
+                             size_t workspace_bytes, void *stream) {
+  return spmm_impl(A, nullptr, b_dtype, ldb, N, nullptr, ldc, plan, workspace,
+                   workspace_bytes, static_cast<hipStream_t>(stream), nullptr, kPhasePrepare);
+}
+
+int32_t segtaco_spmm_prepared(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
+                              int64_t ldb, int32_t N, float *C, int64_t ldc,
+                              const segtaco_plan_t *plan, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+  return spmm_impl(A, B, b_dtype, ldb, N, C, ldc, plan, workspace, workspace_bytes,
+                   static_cast<hipStream_t>(stream), nullptr, kPhaseRun);
 }
 
 int32_t segtaco_spmm_stage_times(const segtaco_csr_t *A, const void *B,

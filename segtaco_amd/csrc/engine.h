@@ -60,12 +60,22 @@ constexpr int32_t kMaxNnz = 0x7fffffff - (1 << 23);
 // Compute units of the current device (256 on MI355X; queried once).
 int32_t device_cus();
 
+// Phases of one product: the steps that depend on A alone (kPhasePrepare:
+// row partition and tile rows; eb_cs's slot plan and column-sorted records)
+// and the product itself (kPhaseRun).
+constexpr int32_t kPhasePrepare = 1;
+constexpr int32_t kPhaseRun = 2;
+constexpr int32_t kPhaseAll = kPhasePrepare | kPhaseRun;
+
 // The device API's implementation (engine.cpp). ev: 4 optional events
-// recorded around partition, main kernel and fix-up.
+// recorded around partition, main kernel and fix-up. phases: kPhaseAll for
+// segtaco_spmm; kPhasePrepare alone (B and C unused, may be null) for
+// segtaco_spmm_prepare; kPhaseRun alone for segtaco_spmm_prepared, which
+// requires the workspace's last preparation to be of this product.
 int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
                   int64_t ldb, int32_t N, float *C, int64_t ldc,
                   const segtaco_plan_t *plan, void *ws, size_t ws_bytes,
-                  hipStream_t stream, hipEvent_t *ev);
+                  hipStream_t stream, hipEvent_t *ev, int32_t phases = kPhaseAll);
 size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
                                  const segtaco_plan_t *plan);
 // Resolves open plan fields without operand pointers (ld's as given).

@@ -11,25 +11,32 @@
 //
 // How, keeping the reference's arithmetic (spmm_eb_sr.h:5-78 semantics,
 // Eigen's per-row sequential order):
-//   * tiles are the eb_sr tiles (T consecutive entries, the K21 partition
-//     and its repaired first row, k_block_starts); a unit is G consecutive
-//     tiles, one per lane group of one 512-thread workgroup;
-//   * once per call, k_cs_count and k_cs_sort (one wave per tile) classify
-//     each entry's row as head (began in an earlier tile), tail (continues
-//     into a later tile) or interior, number the unit's interior rows (LDS
-//     slots), and write each tile's entries sorted by (column, position)
-//     with their slot codes;
+//   * tiles are the eb_sr tiles (T consecutive entries and the repaired
+//     first row of each, k_tile_rows_fix); a unit is G consecutive tiles,
+//     one per lane group of one 512-thread workgroup, eight per wave;
+//   * a preparation pass, k_cs_count then k_cs_sort (one wave per tile),
+//     decides per wave of the unit whether its tiles sweep: they do when
+//     their interior rows (rows that start and end inside a tile) fit the
+//     LDS slots beside those of the unit's earlier sweeping waves. A
+//     sweeping tile's rows get LDS slots — slot 2g for the row tile g
+//     continues from an earlier tile (head), 2g + 1 for the row it leaves
+//     open for a later one (tail), 2G + k for the unit's k-th interior row —
+//     and its entries are written sorted by (column, position) as one record
+//     each (column, slot, value). A walking tile's entries stay in CSR order
+//     and name their row by its place in the tile (head, tail, k-th interior
+//     row, whose row id goes to a side array);
 //   * k_eb_cs, per unit and per column panel of W = 8*V columns: zero the
-//     unit's LDS accumulators, let every lane group sweep its sorted tile —
-//     interior rows accumulate in LDS, head and tail rows in registers —
-//     then write the unit's interior rows to C and the head/tail partials to
-//     the carry arrays; as for eb_sr, k_tile_rows_fix (k_eb.hip) zeroes the
-//     empty rows and k_fixup (sched_eb.h) combines the carries.
-// Each row's entries still meet its accumulator in ascending column order,
-// which is CSR order for canonical CSR, with one fmaf per entry starting from
-// zero: C is bit-identical to eb_sr with the same tile size. A unit whose
-// interior rows exceed the LDS capacity runs its tiles in CSR order with
-// register accumulators instead (same arithmetic).
+//     unit's LDS slots, let every sweeping lane group run its sorted tile —
+//     per entry one 16-byte gather per lane and one read-modify-write of its
+//     row's slot, with no branch on the kind of row — and every walking group
+//     run its tile in CSR order with register accumulators, written out at
+//     each change of row; then write interior slots to C and head / tail
+//     slots to the carry arrays. k_tile_rows_fix zeroes the empty rows and
+//     k_fixup combines the carries, as for eb_sr.
+// Each row's entries meet its accumulator in ascending column order (sweep)
+// or CSR order (walk), the same order for canonical CSR, with one fmaf per
+// entry starting from zero: C is bit-identical to eb_sr with the same tile
+// size. Walking is decided per wave so that no wave runs both paths.
 #include <climits>
 
 #include "cs.h"
@@ -40,6 +47,16 @@ namespace segtaco {
 namespace {
 
 constexpr int32_t kCsPosSlack = 8;
+constexpr int kCsPrepWaves = kCsPrepThreads / 64;
+// Bits of an entry's place in its tile inside a sort key (kCsMaxTile = 2^9).
+constexpr int kCsIdxBits = 9;
+static_assert((1 << kCsIdxBits) == kCsMaxTile, "sort key index field");
+
+// Record of one entry when columns do not fit the packed form (one 12-byte
+// load).
+struct CsWide {
+  uint32_t col, code, vbits;
+};
 
 // Row of entry p given the staged window s_pos[0..pw) = pos[rbase..] (falls
 // back to a bisection of pos in [rbase + pw - 1, r_hi] past the window).
@@ -64,14 +81,6 @@ __device__ __forceinline__ int32_t cs_P(const int32_t *s_pos, int32_t pw,
   return i < pw ? s_pos[i] : pos[row];
 }
 
-// Per-wave LDS of k_cs_prep (tile <= kCsMaxTile entries).
-struct PrepLds {
-  uint64_t key[kCsMaxTile];
-  int32_t pos[kCsMaxTile + kCsPosSlack];
-  int32_t row[kCsMaxTile];
-  int32_t slot[kCsMaxTile];
-};
-
 // Wave-wide inclusive prefix sum of x (64 lanes).
 __device__ __forceinline__ int32_t wave_incl_scan(int32_t x, int lane) {
   for (int d = 1; d < 64; d <<= 1) {
@@ -81,172 +90,478 @@ __device__ __forceinline__ int32_t wave_incl_scan(int32_t x, int lane) {
   return x;
 }
 
-// Tile t's rows (phase A) or rows + slot codes + sorted output (phase B).
-// Returns the tile's interior row count and sets *flags (bit 0 head, bit 1
-// tail). Uniform over the wave.
-template <bool kWrite>
-__device__ int32_t cs_prep_tile(const CsArgs &a, int32_t t, PrepLds &L,
-                                int32_t base, int32_t unit, int32_t *flags) {
+// Wave-wide sum of x.
+__device__ __forceinline__ int32_t wave_sum(int32_t x) {
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// One tile as the prep kernel sees it: each lane holds entries lane + 64k
+// (k < E), their rows, and whether the tile has a head / tail row.
+template <int E>
+struct PrepTile {
+  int32_t s, e, cn;
+  int32_t pw;  // rows staged in the wave's window from r_first
+  int32_t row[E];
+  bool head, tail;
+  int32_t r_first, r_last;
+};
+
+// Rows of tile t's entries and its head / tail rows. s_pos: this wave's
+// LDS window of the row pointer (kCsMaxTile + kCsPosSlack entries).
+template <int E>
+__device__ void cs_tile_rows(const CsArgs &a, int32_t t, int32_t *s_pos, PrepTile<E> &P) {
   const int lane = threadIdx.x & 63;
   const int32_t *__restrict__ pos = a.pos;
-  const int32_t s = t * a.tile;
-  const int32_t e = (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile);
-  const int32_t cn = e - s;
+  P.s = t * a.tile;
+  P.e = (int32_t)min((int64_t)a.nnz, (int64_t)P.s + a.tile);
+  P.cn = P.e - P.s;
   const int32_t rbase = a.tile_row[t];
   const int32_t r_hi = min(a.M - 1, a.tile_row[t + 1]);
   const int32_t pw = min(a.tile + kCsPosSlack, a.M + 1 - rbase);
-  for (int32_t i = lane; i < pw; i += 64) L.pos[i] = pos[rbase + i];
+  P.pw = pw;
+  for (int32_t i = lane; i < pw; i += 64) s_pos[i] = pos[rbase + i];
   wave_sync();
-  for (int32_t i = lane; i < cn; i += 64)
-    L.row[i] = cs_row(L.pos, pw, rbase, pos, r_hi, s + i);
-  wave_sync();
-  const int32_t r_first = L.row[0], r_last = L.row[cn - 1];
-  const bool head = cs_P(L.pos, pw, rbase, pos, r_first) < s;
-  const bool tail = !(head && r_last == r_first) &&
-                    cs_P(L.pos, pw, rbase, pos, r_last + 1) > e;
-  // interior rank: starts in [s, p] minus one, entries processed in order
-  // in chunks of 64 (one per lane) with a running total
-  int32_t total = 0;
-  for (int32_t c0 = 0; c0 < cn; c0 += 64) {
-    const int32_t i = c0 + lane;
-    const bool ok = i < cn;
-    const int32_t r = ok ? L.row[i] : -1;
-    const int32_t st = ok && cs_P(L.pos, pw, rbase, pos, r) == s + i ? 1 : 0;
-    const int32_t inc = wave_incl_scan(st, lane) + total;
-    if (kWrite && ok) {
-      int32_t code;
-      if (head && r == r_first)
-        code = kCsHead;
-      else if (tail && r == r_last)
-        code = kCsTail;
-      else
-        code = base + inc - 1;
-      L.slot[i] = code;
-      if (st && code < kCsTail && code < a.cap)
-        a.slot_rows[(int64_t)unit * a.cap + code] = r;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int32_t i = lane + 64 * k;
+    P.row[k] = i < P.cn ? cs_row(s_pos, pw, rbase, pos, r_hi, P.s + i) : INT_MAX;
+  }
+  P.r_first = rbase;  // the repaired row of the tile's first entry
+  // the row of the last entry: the lane / register holding entry cn - 1
+  const int32_t last = P.cn - 1;
+  int32_t rl = 0;
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    if (last / 64 == k) rl = __shfl(P.row[k], last & 63, 64);
+  P.r_last = rl;
+  P.head = cs_P(s_pos, pw, rbase, pos, P.r_first) < P.s;
+  P.tail = !(P.head && P.r_last == P.r_first) && cs_P(s_pos, pw, rbase, pos, P.r_last + 1) > P.e;
+  // callers read s_pos again: it stays valid until the next tile
+}
+
+// Interior rows of the tile: those that start and end inside it. Entry i
+// starts one when its row begins at i and is neither the head nor the tail.
+template <int E>
+__device__ __forceinline__ bool cs_starts_interior(const CsArgs &a, const PrepTile<E> &P,
+                                                   const int32_t *s_pos, int k) {
+  const int lane = threadIdx.x & 63;
+  const int32_t i = lane + 64 * k;
+  if (i >= P.cn) return false;
+  const int32_t r = P.row[k];
+  if (cs_P(s_pos, P.pw, P.r_first, a.pos, r) != P.s + i) return false;
+  if (P.head && r == P.r_first) return false;
+  if (P.tail && r == P.r_last) return false;
+  return true;
+}
+
+// Slot plan of tile t's unit. The main kernel's wave w runs tiles 8w ..
+// 8w + 7 of the unit; in wave order, a wave sweeps when the interior rows of
+// its tiles fit the slots left after the 2G head / tail slots and the
+// earlier sweeping waves' rows, and walks otherwise (a later, smaller wave
+// may still sweep). Returns tile t's first interior slot, or -1 when its
+// wave walks; n_slots = slots the unit uses.
+__device__ int32_t cs_slot_base(const CsArgs &a, int32_t t, int32_t &n_slots) {
+  const int lane = threadIdx.x & 63;
+  const int32_t t0 = (t / a.group) * a.group;
+  const int32_t g = t - t0;
+  const int32_t cnt = (lane < a.group && t0 + lane < a.ntiles) ? a.tile_count[t0 + lane] : 0;
+  // interior rows of the tiles before g in g's wave, and of each wave
+  const int32_t w_g = g >> 3;
+  const int32_t before = wave_sum((lane >> 3) == w_g && lane < g ? cnt : 0);
+  int32_t seg = cnt;
+  for (int d = 4; d > 0; d >>= 1) seg += __shfl_xor(seg, d, 64);
+  const int32_t room = a.cap - 2 * a.group;
+  int32_t used = 0, base = -1;
+  for (int w = 0; w < 8; ++w) {
+    const int32_t sw = __shfl(seg, 8 * w, 64);
+    if (used + sw <= room) {
+      if (w == w_g) base = 2 * a.group + used + before;
+      used += sw;
     }
-    total = __shfl(inc, 63, 64);
   }
-  *flags = (head ? 1 : 0) | (tail ? 2 : 0);
-  if (!kWrite) {
-    wave_sync();
-    return total - (tail ? 1 : 0);
-  }
-  // sort by (column, position): bitonic over the next power of two
-  int32_t n2 = 1;
-  while (n2 < cn) n2 <<= 1;
-  const int32_t *__restrict__ crd = a.crd;
-  for (int32_t i = lane; i < n2; i += 64)
-    L.key[i] = i < cn ? ((uint64_t)(uint32_t)crd[s + i] << 32) | (uint32_t)i
-                      : ~(uint64_t)0;
-  wave_sync();
-  for (int32_t k = 2; k <= n2; k <<= 1) {
-    for (int32_t j = k >> 1; j > 0; j >>= 1) {
-      for (int32_t i = lane; i < n2; i += 64) {
-        const int32_t l = i ^ j;
-        if (l > i) {
-          const uint64_t x = L.key[i], y = L.key[l];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) {
-            L.key[i] = y;
-            L.key[l] = x;
-          }
+  n_slots = 2 * a.group + used;
+  return base;
+}
+
+// Ascending bitonic sort of the 64 * R keys held as key[r] of lane l =
+// element 64r + l. Distances of 64 and more pair registers of one lane;
+// shorter ones pair lanes through a shuffle.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint32_t (&key)[R]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const int jr = j / 64;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if ((r & jr) != 0) continue;
+          const int rp = r | jr;
+          // element 64r + l is below its partner; the pair is ascending
+          // when bit k of the element index is clear (k >= 128 here)
+          const bool asc = ((64 * r) & k) == 0;
+          const uint32_t lo = min(key[r], key[rp]), hi = max(key[r], key[rp]);
+          key[r] = asc ? lo : hi;
+          key[rp] = asc ? hi : lo;
+        }
+      } else {
+        const bool lower = (lane & j) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t p = (uint32_t)__shfl_xor((int32_t)key[r], j, 64);
+          const bool asc = (((64 * r) | lane) & k) == 0;
+          key[r] = (asc == lower) ? min(key[r], p) : max(key[r], p);
         }
       }
-      wave_sync();
     }
   }
-  for (int32_t j = lane; j < cn; j += 64) {
-    const uint64_t key = L.key[j];
-    const int32_t i = (int32_t)(key & 0xffffffffu);
-    a.s_col[s + j] = (int32_t)(key >> 32);
-    a.s_slot[s + j] = (uint16_t)L.slot[i];
-    if (a.vals) {
-      if (a.elem == 4)
-        static_cast<uint32_t *>(a.s_val)[s + j] =
-            static_cast<const uint32_t *>(a.vals)[s + i];
-      else
-        static_cast<uint16_t *>(a.s_val)[s + j] =
-            static_cast<const uint16_t *>(a.vals)[s + i];
-    }
-  }
-  wave_sync();
-  return total - (tail ? 1 : 0);
 }
 
-// Phase A, one wave per tile: interior row count and head/tail flags.
+// Prep, one wave per tile (many waves per CU hide the latency of the
+// per-entry row searches). k_cs_count: the tile's interior rows and its
+// head / tail flags. k_cs_sort: the unit's slot plan (cs_slot_base), the
+// code of every entry's row — LDS slot when sweeping; 0 head, 1 tail, 2 + k
+// for the k-th interior row when walking — and the rows behind the codes,
+// then the records: sweeping tiles sorted by (column, position), walking
+// tiles in CSR order.
+template <int E>
 __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
-  __shared__ PrepLds lds[kCsPrepThreads / 64];
+  __shared__ int32_t s_pos[kCsPrepWaves][kCsMaxTile + kCsPosSlack];
+  const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int32_t t = blockIdx.x * (kCsPrepThreads / 64) + wave;
+  const int32_t t = blockIdx.x * kCsPrepWaves + wave;
   if (t >= a.ntiles) return;  // uniform over the wave
-  int32_t fl;
-  const int32_t n = cs_prep_tile<false>(a, t, lds[wave], 0, 0, &fl);
-  if ((threadIdx.x & 63) == 0) {
+  int32_t *sp = s_pos[wave];
+  PrepTile<E> P;
+  cs_tile_rows<E>(a, t, sp, P);
+  int32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < E; ++k) n += cs_starts_interior<E>(a, P, sp, k) ? 1 : 0;
+  n = wave_sum(n);
+  if (lane == 0) {
     a.tile_count[t] = n;
-    a.tile_flags[t] = (uint8_t)fl;
+    a.tile_flags[t] = (uint8_t)((P.head ? 1 : 0) | (P.tail ? 2 : 0));
   }
 }
 
-// Phase B, one wave per tile: the tile's first slot is the sum of the counts
-// of the unit's earlier tiles; slot codes, slot rows and the sorted copy.
+template <int E>
 __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
-  __shared__ PrepLds lds[kCsPrepThreads / 64];
-  const int wave = threadIdx.x >> 6;
+  constexpr int R = E <= 1 ? 1 : E <= 2 ? 2 : E <= 4 ? 4 : 8;  // sort registers
+  __shared__ int32_t s_pos[kCsPrepWaves][kCsMaxTile + kCsPosSlack];
+  __shared__ int32_t s_code[kCsPrepWaves][kCsMaxTile];
   const int lane = threadIdx.x & 63;
-  const int32_t t = blockIdx.x * (kCsPrepThreads / 64) + wave;
+  const int wave = threadIdx.x >> 6;
+  const int32_t t = blockIdx.x * kCsPrepWaves + wave;
   if (t >= a.ntiles) return;  // uniform over the wave
+  int32_t *sp = s_pos[wave];
+  int32_t *sc = s_code[wave];
   const int32_t unit = t / a.group;
-  const int32_t t0 = unit * a.group;
-  int32_t part = 0;
-  for (int32_t i = t0 + lane; i < t; i += 64) part += a.tile_count[i];
-  for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
-  int32_t fl;
-  cs_prep_tile<true>(a, t, lds[wave], part, unit, &fl);
-  const bool last = t == a.ntiles - 1 || t == t0 + a.group - 1;
-  if (last && lane == 0) a.n_slots[unit] = part + a.tile_count[t];
+  const int32_t g = t - unit * a.group;
+  int32_t n_slots = 0;
+  const int32_t base = cs_slot_base(a, t, n_slots);
+  const bool walk = base < 0;
+  if (g == 0 && lane == 0) a.n_slots[unit] = n_slots;
+
+  PrepTile<E> P;
+  cs_tile_rows<E>(a, t, sp, P);
+  if (walk && lane == 0) a.tile_flags[t] |= 4;
+  const int32_t *__restrict__ crd = a.crd;
+  int32_t code[E], col[E];
+  int32_t running = 0;  // interior rows started by earlier chunks
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int32_t i = lane + 64 * k;
+    const bool st = cs_starts_interior<E>(a, P, sp, k);
+    const int32_t inc = wave_incl_scan(st ? 1 : 0, lane) + running;
+    running = __shfl(inc, 63, 64);
+    // the entry that starts an interior row names the row's place k in the
+    // tile; the row's other entries take it from the running maximum below
+    code[k] = st ? inc - 1 : -1;
+    if (st) {
+      if (walk)
+        a.walk_rows[(int64_t)t * a.tile + inc - 1] = P.row[k];
+      else
+        a.slot_rows[(int64_t)unit * a.cap + base + inc - 1] = P.row[k];
+    }
+    col[k] = i < P.cn ? crd[P.s + i] : INT_MAX;
+  }
+  int32_t carry = -1;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    int32_t m = code[k];
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t y = __shfl_up(m, d, 64);
+      if (lane >= d) m = max(m, y);
+    }
+    m = max(m, carry);
+    carry = __shfl(m, 63, 64);
+    const int32_t r = P.row[k];
+    const bool hd = P.head && r == P.r_first;
+    const bool tl = !hd && P.tail && r == P.r_last;
+    if (walk)
+      code[k] = hd ? 0 : tl ? 1 : 2 + m;
+    else
+      code[k] = hd ? 2 * g : tl ? 2 * g + 1 : base + m;
+  }
+  const uint32_t one = a.one_bits;  // A without values reads 1.0
+  // record of entry i (its place in the tile) at sorted place d
+  auto put = [&](int32_t d, int32_t i, int32_t c, int32_t cd) {
+    uint32_t vb = one;
+    if (a.vals)
+      vb = a.elem == 4 ? static_cast<const uint32_t *>(a.vals)[P.s + i]
+                       : (uint32_t) static_cast<const uint16_t *>(a.vals)[P.s + i];
+    const int64_t dst = (int64_t)P.s + d;
+    if (a.packed)
+      static_cast<uint2 *>(a.s_rec)[dst] =
+          make_uint2(((uint32_t)c << kCsCodeBits) | (uint32_t)cd, vb);
+    else
+      static_cast<CsWide *>(a.s_rec)[dst] = CsWide{(uint32_t)c, (uint32_t)cd, vb};
+  };
+  if (walk) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int32_t i = lane + 64 * k;
+      if (i < P.cn) put(i, i, col[k], code[k]);
+    }
+    return;
+  }
+  if (a.packed) {
+    // columns below 2^(32 - kCsCodeBits) <= 2^20: the key (column << 9 |
+    // place) is unique, fits 29 bits and orders by (column, position);
+    // places past the tile sort last
+    uint32_t key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t i = lane + 64 * r;
+      key[r] = 0xffffffffu;
+      if (r < E && i < P.cn) {
+        key[r] = ((uint32_t)col[r] << kCsIdxBits) | (uint32_t)i;
+        sc[i] = code[r];
+      }
+    }
+    wave_sync();
+    wave_bitonic<R>(key);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int32_t d = lane + 64 * r;
+      if (d < P.cn) {
+        const int32_t i = (int32_t)(key[r] & (kCsMaxTile - 1));
+        put(d, i, (int32_t)(key[r] >> kCsIdxBits), sc[i]);
+      }
+    }
+    return;
+  }
+  // wide columns: rank of each entry in (column, position) order
+  int32_t rank[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) rank[k] = 0;
+#pragma unroll
+  for (int k2 = 0; k2 < E; ++k2) {
+    if (64 * k2 >= P.cn) break;  // uniform
+    const int32_t lim = min(64, P.cn - 64 * k2);
+    for (int32_t l = 0; l < lim; ++l) {
+      const int32_t cj = __builtin_amdgcn_readlane(col[k2], l);
+      const int32_t j = 64 * k2 + l;
+#pragma unroll
+      for (int k = 0; k < E; ++k)
+        rank[k] += (cj < col[k] || (cj == col[k] && j < lane + 64 * k)) ? 1 : 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int32_t i = lane + 64 * k;
+    if (i < P.cn) put(rank[k], i, col[k], code[k]);
+  }
 }
 
-// CSR-order walk of tile t, panel columns [col, col + V): the fallback for a
-// unit whose interior rows do not fit the LDS accumulators.
-template <int DT, int V>
-__device__ void cs_fallback_tile(const CsArgs &a, int32_t t, int32_t col) {
-  const int32_t *__restrict__ pos = a.pos;
-  const int32_t s = t * a.tile;
-  const int32_t e = (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile);
-  int32_t r = a.tile_row[t];
-  int32_t rs = pos[r], re = pos[r + 1];
+// Record of sorted entry i: column, slot and value bits, one load.
+template <bool kPacked>
+__device__ __forceinline__ void cs_rec(const void *rec, int32_t i, int32_t &col,
+                                       int32_t &code, uint32_t &vbits) {
+  if constexpr (kPacked) {
+    const uint2 w = static_cast<const uint2 *>(rec)[i];
+    col = (int32_t)(w.x >> kCsCodeBits);
+    code = (int32_t)(w.x & ((1u << kCsCodeBits) - 1));
+    vbits = w.y;
+  } else {
+    const CsWide w = static_cast<const CsWide *>(rec)[i];
+    col = (int32_t)w.col;
+    code = (int32_t)w.code;
+    vbits = w.vbits;
+  }
+}
+
+// Value bits (FP32, or 16-bit storage zero-extended) widened to FP32.
+template <int DT>
+__device__ __forceinline__ float cs_widen(uint32_t r) {
+  if constexpr (DT == 0)
+    return __uint_as_float(r);
+  else
+    return half_to_f32<DT>(r);
+}
+
+// Entries per batch: metadata of batch b + 2, gathers of batch b + 1 and
+// the accumulation of batch b are in flight together. Wide records hold
+// three words, so their batches are smaller (all buffers stay in registers).
+template <bool kPacked>
+constexpr int cs_batch() {
+  return kPacked ? 8 : 6;
+}
+
+template <int NB>
+struct CsBatch {
+  static constexpr int kN = NB;
+  int32_t c[NB], s[NB];
+  uint32_t v[NB];
+};
+
+// The pipelined loop shared by the sweep and the walk over entries [s, e):
+// Every load is unconditional (entries past the tile re-read its last
+// entry; columns past N read column 0; neither is used), so the loads in
+// flight are the same on every path and each wait covers only its batch:
+// loads complete in issue order, and per iteration the metadata of batch
+// b + 2 goes out before the gathers of batch b + 1, so waiting for the
+// metadata those gathers need (issued an iteration earlier) leaves the
+// gathers of batch b in flight. Three metadata buffers and two gather
+// buffers rotate by name through an unrolled cycle of six steps: a copy of a
+// register whose load is still in flight would make the wave wait for that
+// load, so no buffer is copied. A step consumes batch b (metadata X, gathers
+// GX), gathers batch b + 1 (metadata Y into GY) and loads the metadata of
+// batch b + 2 into Z. G is the gather buffer type, filled by gather(m, g)
+// and read by consume(m, g, i0).
+template <bool kPacked, typename M, typename G, typename Gather, typename Consume>
+__device__ __forceinline__ void cs_pipeline(const CsArgs &a, int32_t s, int32_t e,
+                                            Gather &&gather, Consume &&consume) {
+  if (s >= e) return;
+  const int32_t last = e - 1;
+  constexpr int kB = M::kN;
+  auto load_meta = [&](M &m, int32_t i0) {
+#pragma unroll
+    for (int u = 0; u < kB; ++u)
+      cs_rec<kPacked>(a.s_rec, min(i0 + u, last), m.c[u], m.s[u], m.v[u]);
+  };
+  M mA, mB, mC;
+  G gA, gB;
+  load_meta(mA, s);
+  load_meta(mB, s + kB);  // past the tile: clamped, unused
+  gather(mA, gA);
+  int32_t i0 = s;
+  auto step = [&](M &X, G &GX, M &Y, G &GY, M &Z) {
+    load_meta(Z, i0 + 2 * kB);
+    gather(Y, GY);
+    consume(X, GX, i0);
+    i0 += kB;
+    return i0 < e;
+  };
+  for (;;) {
+    if (!step(mA, gA, mB, gB, mC)) break;
+    if (!step(mB, gB, mC, gA, mA)) break;
+    if (!step(mC, gA, mA, gB, mB)) break;
+    if (!step(mA, gB, mB, gA, mC)) break;
+    if (!step(mB, gA, mC, gB, mA)) break;
+    if (!step(mC, gB, mA, gA, mB)) break;
+  }
+}
+
+// Sweep of a sorted tile: each entry's product added to its row's LDS slot.
+template <int DT, int V, bool kPacked>
+__device__ __forceinline__ void cs_sweep(const CsArgs &a, float *acc_lds, int32_t s,
+                                         int32_t e, int32_t col, bool colok) {
+  constexpr int32_t Q = kCsLanes;
+  constexpr int32_t W = Q * V;
+  const int32_t q = (threadIdx.x & 63) & (Q - 1);
+  constexpr int kB = cs_batch<kPacked>();
+  using M = CsBatch<kB>;
+  using R = typename RawT<DT, V>::type;
+  struct G {
+    R b[kB];
+  };
+  const int32_t col_ld = colok ? col : 0;
+  auto gather = [&](const M &m, G &g) {
+#pragma unroll
+    for (int u = 0; u < kB; ++u)
+      g.b[u] = load_raw<DT, V>(a.B, (int64_t)m.c[u] * a.ldb + col_ld);
+  };
+  auto consume = [&](const M &m, const G &g, int32_t i0) {
+    const int32_t nb = min(kB, e - i0);
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (u < nb && colok) {
+        const float v = cs_widen<DT>(m.v[u]);
+        float b[V];
+        unpack<DT, V>(g.b[u], b);
+        float *p = acc_lds + m.s[u] * W + q * V;
+        float x[V];
+        load_f32vec<V>(p, x);
+#pragma unroll
+        for (int k = 0; k < V; ++k) x[k] = fmaf(v, b[k], x[k]);
+        store_vec<V>(p, x);
+      }
+    }
+  };
+  cs_pipeline<kPacked, M, G>(a, s, e, gather, consume);
+}
+
+// Walk of a tile in CSR order (its wave did not fit the LDS slots): a
+// register accumulator per row, stored when the row code changes — code 0
+// to the head carry, 1 to the tail carry, 2 + k to C's row walk_rows[k] of
+// the tile (loaded with the gathers, so it is in registers by then).
+template <int DT, int V, bool kPacked>
+__device__ __forceinline__ void cs_walk(const CsArgs &a, int32_t t, int32_t s, int32_t e,
+                                        int32_t col, bool colok) {
+  constexpr int kB = cs_batch<kPacked>();
+  using M = CsBatch<kB>;
+  using R = typename RawT<DT, V>::type;
+  struct G {
+    R b[kB];
+    int32_t row[kB];
+  };
+  const int32_t col_ld = colok ? col : 0;
+  const int32_t *rows = a.walk_rows + (int64_t)t * a.tile;
+  auto gather = [&](const M &m, G &g) {
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      g.b[u] = load_raw<DT, V>(a.B, (int64_t)m.c[u] * a.ldb + col_ld);
+      g.row[u] = rows[max(m.s[u] - 2, 0)];
+    }
+  };
   float acc[V];
   zero_vec(acc);
-  const bool colok = col < a.N;
+  int32_t cur = -1, row = 0;
   auto flush = [&]() {
-    if (!colok) return;
-    float *dst = rs < s   ? a.carry_head + (int64_t)t * a.N + col
-                 : re > e ? a.carry_tail + (int64_t)t * a.N + col
-                          : a.C + (int64_t)r * a.ldc + col;
+    if (cur < 0 || !colok) return;
+    float *dst = cur == 0   ? a.carry_head + (int64_t)t * a.N + col
+                 : cur == 1 ? a.carry_tail + (int64_t)t * a.N + col
+                            : a.C + (int64_t)row * a.ldc + col;
     store_vec<V>(dst, acc);
   };
-  for (int32_t p = s; p < e; ++p) {
-    if (p == re) {
-      flush();
-      ++r;
-      while (pos[r + 1] <= p) ++r;
-      rs = pos[r];
-      re = pos[r + 1];
-      zero_vec(acc);
-    }
-    if (colok) {
-      const float v = load_val<DT>(a.vals, p);
-      float b[V];
-      unpack<DT, V>(load_raw<DT, V>(a.B, (int64_t)a.crd[p] * a.ldb + col), b);
+  auto consume = [&](const M &m, const G &g, int32_t i0) {
+    const int32_t nb = min(kB, e - i0);
 #pragma unroll
-      for (int j = 0; j < V; ++j) acc[j] = fmaf(v, b[j], acc[j]);
+    for (int u = 0; u < kB; ++u) {
+      if (u < nb) {
+        if (m.s[u] != cur) {
+          flush();
+          cur = m.s[u];
+          row = g.row[u];
+          zero_vec(acc);
+        }
+        const float v = cs_widen<DT>(m.v[u]);
+        float b[V];
+        unpack<DT, V>(g.b[u], b);
+#pragma unroll
+        for (int k = 0; k < V; ++k) acc[k] = fmaf(v, b[k], acc[k]);
+      }
     }
-  }
+  };
+  cs_pipeline<kPacked, M, G>(a, s, e, gather, consume);
   flush();
 }
 
-template <int DT, int V>
+template <int DT, int V, bool kPacked>
 __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
   extern __shared__ float acc_lds[];  // [cap][W]
   constexpr int32_t Q = kCsLanes;
@@ -254,9 +569,7 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int32_t q = lane & (Q - 1);
-  const int32_t gbase = lane & ~(Q - 1);        // first lane of my group
   const int32_t g = wave * (64 / Q) + (lane / Q);  // group in workgroup
-  using R = typename RawT<DT, V>::type;
   const int32_t panels = (a.N + W - 1) / W;
 
   for (int32_t round = 0; round < a.rounds; ++round) {
@@ -265,130 +578,47 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
     const int32_t t = unit * a.group + g;
     const bool tok = g < a.group && t < a.ntiles;
     const int32_t n_slots = a.n_slots[unit];
-    const bool fits = n_slots <= a.cap;
     const int32_t s = tok ? t * a.tile : 0;
     const int32_t e = tok ? (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile) : 0;
     const int32_t flags = tok ? a.tile_flags[t] : 0;
+    const bool walk = (flags & 4) != 0;  // the same for the wave's tiles
     for (int32_t panel = 0; panel < panels; ++panel) {
       const int32_t col = panel * W + q * V;
       const bool colok = col < a.N;
-      if (!fits) {
-        if (tok) cs_fallback_tile<DT, V>(a, t, col);
-        continue;
-      }
-      for (int32_t i = threadIdx.x; i < n_slots * W; i += kCsThreads) acc_lds[i] = 0.0f;
+      for (int32_t i = threadIdx.x; i < n_slots * W / 4; i += kCsThreads)
+        reinterpret_cast<float4 *>(acc_lds)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
       __syncthreads();
-      float head[V], tail[V];
-      zero_vec(head);
-      zero_vec(tail);
-      // Software pipeline over chunks of Q*Q entries: lane q holds the
-      // metadata of entries c0 + Q*k + q (k < Q); batch k of a chunk is
-      // entries c0 + Q*k .. c0 + Q*k + Q - 1, broadcast from lane u of the
-      // group. The next chunk's metadata loads when a chunk starts, and
-      // batch k+1's gathers are in flight while batch k is consumed.
-      constexpr int32_t CK = Q * Q;
-      int32_t mc[Q], ms[Q], nc[Q], ns[Q];
-      float mv[Q], nv[Q];
-      // Every load below is unconditional (entries past the tile re-read its
-      // last entry, columns past N column 0, neither used), so the number of
-      // loads in flight is the same on every path and each wait covers only
-      // the batch it needs. Called only while s < e.
-      const int32_t col_ld = colok ? col : 0;
-      auto load_meta = [&](int32_t c0, int32_t (&c)[Q], float (&v)[Q], int32_t (&sl)[Q]) {
-#pragma unroll
-        for (int32_t k = 0; k < Q; ++k) {
-          const int32_t jj = c0 + Q * k + q;
-          const int32_t jc = min(jj, e - 1);
-          const bool ok = jj < e;
-          c[k] = a.s_col[jc];
-          const float vk = load_val<DT>(a.s_val, jc);
-          const int32_t sk = (int32_t)a.s_slot[jc];
-          v[k] = ok ? vk : 0.0f;
-          sl[k] = ok ? sk : -1;
-        }
-      };
-      auto issue = [&](R (&buf)[Q], int32_t cq, int32_t) {
-#pragma unroll
-        for (int32_t u = 0; u < Q; ++u) {
-          const int32_t cu = __shfl(cq, gbase + u, 64);
-          buf[u] = load_raw<DT, V>(a.B, (int64_t)cu * a.ldb + col_ld);
-        }
-      };
-      auto consume = [&](const R (&buf)[Q], float vq, int32_t sq, int32_t j0) {
-#pragma unroll
-        for (int32_t u = 0; u < Q; ++u) {
-          const float v = __shfl(vq, gbase + u, 64);
-          const int32_t sl = __shfl(sq, gbase + u, 64);
-          if (j0 + u < e && colok) {
-            float b[V];
-            unpack<DT, V>(buf[u], b);
-            if (sl < kCsTail) {
-              float *p = acc_lds + sl * W + q * V;
-              float x[V];
-#pragma unroll
-              for (int k = 0; k < V; ++k) x[k] = p[k];
-#pragma unroll
-              for (int k = 0; k < V; ++k) x[k] = fmaf(v, b[k], x[k]);
-#pragma unroll
-              for (int k = 0; k < V; ++k) p[k] = x[k];
-            } else if (sl == kCsHead) {
-#pragma unroll
-              for (int k = 0; k < V; ++k) head[k] = fmaf(v, b[k], head[k]);
-            } else {
-#pragma unroll
-              for (int k = 0; k < V; ++k) tail[k] = fmaf(v, b[k], tail[k]);
-            }
-          }
-        }
-      };
-      R buf0[Q], buf1[Q];
-      if (s < e) {
-        load_meta(s, mc, mv, ms);
-        issue(buf0, mc[0], s);
-      }
-      for (int32_t c0 = s; c0 < e; c0 += CK) {
-        const bool more = c0 + CK < e;
-        load_meta(c0 + CK, nc, nv, ns);  // past the tile: clamped, unused
-#pragma unroll
-        for (int32_t k = 0; k < Q; ++k) {
-          const int32_t j0 = c0 + Q * k;
-          // batch k+1 (the next chunk's batch 0 after the last) goes out first
-          if (k + 1 < Q) {
-            if (k & 1)
-              issue(buf0, mc[k + 1], j0 + Q);
-            else
-              issue(buf1, mc[k + 1], j0 + Q);
-          } else {
-            issue(buf0, nc[0], c0 + CK);  // Q is even: batch 0 uses buf0
-          }
-          if (k & 1)
-            consume(buf1, mv[k], ms[k], j0);
-          else
-            consume(buf0, mv[k], ms[k], j0);
-        }
-        if (more) {
-#pragma unroll
-          for (int32_t k = 0; k < Q; ++k) {
-            mc[k] = nc[k];
-            mv[k] = nv[k];
-            ms[k] = ns[k];
-          }
-        }
-      }
-      if (tok && colok) {
-        if (flags & 1) store_vec<V>(a.carry_head + (int64_t)t * a.N + col, head);
-        if (flags & 2) store_vec<V>(a.carry_tail + (int64_t)t * a.N + col, tail);
-      }
+      if (walk)
+        cs_walk<DT, V, kPacked>(a, t, s, e, col, colok);
+      else
+        cs_sweep<DT, V, kPacked>(a, acc_lds, s, e, col, colok);
       __syncthreads();
+      // head / tail slots of sweeping tiles to the carries
+      if (tok && colok && !walk) {
+        float x[V];
+        if (flags & 1) {
+#pragma unroll
+          for (int k = 0; k < V; ++k) x[k] = acc_lds[(2 * g) * W + q * V + k];
+          store_vec<V>(a.carry_head + (int64_t)t * a.N + col, x);
+        }
+        if (flags & 2) {
+#pragma unroll
+          for (int k = 0; k < V; ++k) x[k] = acc_lds[(2 * g + 1) * W + q * V + k];
+          store_vec<V>(a.carry_tail + (int64_t)t * a.N + col, x);
+        }
+      }
+      // interior slots to C
       constexpr int32_t PER = W / V;  // vectors per slot
-      for (int32_t i = threadIdx.x; i < n_slots * PER; i += kCsThreads) {
-        const int32_t sl = i / PER;
-        const int32_t c = panel * W + (i - sl * PER) * V;
+      const int32_t first = 2 * a.group;
+      for (int32_t i = threadIdx.x; i < (n_slots - first) * PER; i += kCsThreads) {
+        const int32_t sl = first + i / PER;
+        const int32_t seg = i % PER;
+        const int32_t c = panel * W + seg * V;
         if (c < a.N) {
           const int32_t row = a.slot_rows[(int64_t)unit * a.cap + sl];
           float x[V];
 #pragma unroll
-          for (int k = 0; k < V; ++k) x[k] = acc_lds[sl * W + (i - sl * PER) * V + k];
+          for (int k = 0; k < V; ++k) x[k] = acc_lds[sl * W + seg * V + k];
           store_vec<V>(a.C + (int64_t)row * a.ldc + c, x);
         }
       }
@@ -401,10 +631,24 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
 
 hipError_t launch_cs_prep(const CsArgs &a, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
-  constexpr int kWaves = kCsPrepThreads / 64;
-  const dim3 g((unsigned)((a.ntiles + kWaves - 1) / kWaves)), b(kCsPrepThreads);
-  hipLaunchKernelGGL(k_cs_count, g, b, 0, s, a);
-  hipLaunchKernelGGL(k_cs_sort, g, b, 0, s, a);
+  const dim3 g((unsigned)((a.ntiles + kCsPrepWaves - 1) / kCsPrepWaves)), b(kCsPrepThreads);
+#define SEGTACO_CS_PREP(E_)                                 \
+  case E_:                                                  \
+    hipLaunchKernelGGL(k_cs_count<E_>, g, b, 0, s, a);      \
+    hipLaunchKernelGGL(k_cs_sort<E_>, g, b, 0, s, a);       \
+    break;
+  switch ((a.tile + 63) / 64) {
+    SEGTACO_CS_PREP(1)
+    SEGTACO_CS_PREP(2)
+    SEGTACO_CS_PREP(3)
+    SEGTACO_CS_PREP(4)
+    SEGTACO_CS_PREP(5)
+    SEGTACO_CS_PREP(6)
+    SEGTACO_CS_PREP(7)
+    SEGTACO_CS_PREP(8)
+    default: return hipErrorInvalidValue;
+  }
+#undef SEGTACO_CS_PREP
   return hipGetLastError();
 }
 
@@ -412,32 +656,38 @@ size_t cs_lds_bytes(int32_t vec, int32_t cap) {
   return (size_t)cap * kCsLanes * vec * sizeof(float);
 }
 
-template <int DT, int V>
+template <int DT, int V, bool kPacked>
 hipError_t launch_eb_cs_t(const CsArgs &a, size_t lds, hipStream_t s) {
   // dynamic LDS above the default per-kernel limit has to be declared; the
   // capacity is fixed per device, so once per instantiation is enough
   static const hipError_t declared = hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&k_eb_cs<DT, V>),
+      reinterpret_cast<const void *>(&k_eb_cs<DT, V, kPacked>),
       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (declared != hipSuccess) return declared;
   const unsigned grid = (unsigned)((a.units + a.rounds - 1) / a.rounds);
-  hipLaunchKernelGGL((k_eb_cs<DT, V>), dim3(grid), dim3(kCsThreads), lds, s, a);
+  hipLaunchKernelGGL((k_eb_cs<DT, V, kPacked>), dim3(grid), dim3(kCsThreads), lds, s, a);
   return hipGetLastError();
+}
+
+template <int DT, int V>
+hipError_t launch_eb_cs_p(const CsArgs &a, size_t lds, hipStream_t s) {
+  return a.packed ? launch_eb_cs_t<DT, V, true>(a, lds, s)
+                  : launch_eb_cs_t<DT, V, false>(a, lds, s);
 }
 
 hipError_t launch_eb_cs(int dt, int vec, const CsArgs &a, hipStream_t s) {
   if (a.units <= 0) return hipSuccess;
   const size_t lds = cs_lds_bytes(vec, a.cap);
   switch (dt * 16 + vec) {
-    case 0 * 16 + 1: return launch_eb_cs_t<0, 1>(a, lds, s);
-    case 0 * 16 + 2: return launch_eb_cs_t<0, 2>(a, lds, s);
-    case 0 * 16 + 4: return launch_eb_cs_t<0, 4>(a, lds, s);
-    case 1 * 16 + 1: return launch_eb_cs_t<1, 1>(a, lds, s);
-    case 1 * 16 + 2: return launch_eb_cs_t<1, 2>(a, lds, s);
-    case 1 * 16 + 4: return launch_eb_cs_t<1, 4>(a, lds, s);
-    case 2 * 16 + 1: return launch_eb_cs_t<2, 1>(a, lds, s);
-    case 2 * 16 + 2: return launch_eb_cs_t<2, 2>(a, lds, s);
-    case 2 * 16 + 4: return launch_eb_cs_t<2, 4>(a, lds, s);
+    case 0 * 16 + 1: return launch_eb_cs_p<0, 1>(a, lds, s);
+    case 0 * 16 + 2: return launch_eb_cs_p<0, 2>(a, lds, s);
+    case 0 * 16 + 4: return launch_eb_cs_p<0, 4>(a, lds, s);
+    case 1 * 16 + 1: return launch_eb_cs_p<1, 1>(a, lds, s);
+    case 1 * 16 + 2: return launch_eb_cs_p<1, 2>(a, lds, s);
+    case 1 * 16 + 4: return launch_eb_cs_p<1, 4>(a, lds, s);
+    case 2 * 16 + 1: return launch_eb_cs_p<2, 1>(a, lds, s);
+    case 2 * 16 + 2: return launch_eb_cs_p<2, 2>(a, lds, s);
+    case 2 * 16 + 4: return launch_eb_cs_p<2, 4>(a, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -78,6 +78,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_tile_rows_fix(
   } else if (r == M) {
     rows[ntiles] = M;
   }
+  if (C == nullptr) return;  // tile rows alone (uniform over the launch)
   unsigned long long m = __ballot(empty);
   const int lane = threadIdx.x & 63;
   const int64_t wave_row0 = r - lane;

@@ -24,14 +24,6 @@
 
 namespace segtaco {
 
-int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
-                  int64_t ldb, int32_t N, float *C, int64_t ldc,
-                  const segtaco_plan_t *plan, void *ws, size_t ws_bytes,
-                  hipStream_t stream, hipEvent_t *ev);
-size_t spmm_workspace_bytes_impl(const segtaco_csr_t *A, int32_t N,
-                                 const segtaco_plan_t *plan);
-int32_t plan_resolve_impl(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
-                          int64_t ldb, int64_t ldc, segtaco_plan_t *plan);
 int32_t vendor_spmm(const segtaco_csr_t *A, const float *B, int64_t ldb,
                     int32_t N, float *C, int64_t ldc, int32_t alg,
                     hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);

@@ -108,6 +108,20 @@ def test_workspace_bytes():
     assert S.workspace_bytes(A, 64, S.plan_resolve(A, 64, S.DT_F32, S.make_plan("rb_sr"))) == 0
 
 
+def test_prepared_refuses_a_workspace_not_prepared_for_the_product():
+    """segtaco_spmm_prepared on an eb_cs workspace that holds no preparation
+    of this product is refused on the host, before any device work (the
+    pointers here are never dereferenced)."""
+    A = _desc(M=1000, K=1000, nnz=5000)
+    p = S.plan_resolve(A, 64, S.DT_F32, S.make_plan("eb_cs"))
+    need = S.workspace_bytes(A, 64, p)
+    with pytest.raises(S.SegtacoError, match="not prepared"):
+        S.spmm_prepared(A, 0x10000, S.DT_F32, 64, 64, 0x20000, 64, p, 0x40000, need)
+    # a preparation without a workspace is refused too
+    with pytest.raises(S.SegtacoError, match="workspace"):
+        S.spmm_prepare(A, S.DT_F32, 64, 64, 64, p, None, 0)
+
+
 def test_validate_csr():
     pos = np.array([0, 2, 2, 5], np.int32)
     crd = np.array([0, 3, 1, 2, 4], np.int32)
