@@ -195,11 +195,13 @@ void spmm_csr_gpu_cusparse(taco_tensor_t *C, taco_tensor_t *A, taco_tensor_t *B)
  * the rows of a unit of tiles accumulate in LDS. Same results as eb_sr with
  * the same tile for column-sorted rows; no compensated summation. */
 #define SEGTACO_EB_CS 4
-/* Block-dense, for BF16 / FP16 storage: rows in blocks of 32 (tile = 32);
- * a block whose entries span at most 256 columns is scattered into a dense
- * tile and multiplied with the matrix cores (MFMA, FP32 accumulation),
- * wider blocks run a per-row vector walk. For banded and block-structured
- * A. Runs standalone only (not in batches or the server). */
+/* Block-dense, for BF16 / FP16 storage: rows in blocks of tile = 32
+ * (default) or 64; a 32-row block whose entries span at most 256 columns, or
+ * a 64-row block spanning at most 192 (else each of its 32-row halves as
+ * above), is scattered into a dense tile and multiplied with the matrix
+ * cores (MFMA, FP32 accumulation); wider blocks run a per-row vector walk.
+ * For banded and block-structured A. Runs standalone only (not in batches
+ * or the server). */
 #define SEGTACO_BD 5
 
 /* A schedule and its parameters. Zero fields are resolved by
@@ -215,7 +217,8 @@ typedef struct segtaco_plan {
   int32_t tile;           /* eb_*: nnz per tile (multiple of 8); 0 = auto:
                              eb_sr 128, halved (down to 8) while the product
                              would fill fewer than two workgroups per compute
-                             unit; eb_pr 128 (256 at N = 1) */
+                             unit; eb_pr 128 (256 at N = 1); bd: rows per
+                             block, 32 (auto) or 64 */
   int32_t comp_threshold; /* rows with more nnz than this use compensated
                              (Neumaier) summation; 0 = never; -1 = default
                              (1024 for 16-bit storage, never for FP32) */

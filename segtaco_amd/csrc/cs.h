@@ -47,8 +47,10 @@ struct CsArgs {
                         // tiles: CSR order, code 0 head, 1 tail, 2 + k the
                         // tile's k-th interior row
   int32_t *slot_rows;   // units x cap: row of each interior slot
-  int32_t *walk_rows;   // ntiles x tile: row of a walking tile's k-th
-                        // interior row
+  int32_t *int_rows;    // ntiles x tile: row of each tile's k-th interior
+                        // row (one that starts and ends inside the tile)
+  uint16_t *codes;      // nnz: row of each entry by its place in the tile
+                        // (0 head, 1 tail, 2 + k the k-th interior row)
   int32_t *n_slots;     // units: slots used (2G + swept interior rows)
   int32_t *tile_count;  // ntiles: interior rows of each tile
   uint8_t *tile_flags;  // ntiles: bit 0 head row, bit 1 tail row, bit 2

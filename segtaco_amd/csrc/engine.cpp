@@ -57,7 +57,7 @@ EbLayout eb_layout(int32_t nnz, int32_t N, int32_t tile) {
 struct CsLayout {
   EbLayout eb;
   int32_t units, cap, packed;
-  size_t rec_off, rows_off, walk_off, nslot_off, count_off, flag_off, total;
+  size_t rec_off, rows_off, int_off, code_off, nslot_off, count_off, flag_off, total;
 };
 
 size_t device_max_lds();
@@ -80,7 +80,8 @@ CsLayout cs_layout(int32_t nnz, int32_t K, int32_t N, int32_t tile, int32_t grou
   };
   L.rec_off = take((L.packed ? 8 : 12) * (size_t)nnz);
   L.rows_off = take(sizeof(int32_t) * (size_t)L.units * L.cap);
-  L.walk_off = take(sizeof(int32_t) * (size_t)L.eb.ntiles * tile);
+  L.int_off = take(sizeof(int32_t) * (size_t)L.eb.ntiles * tile);
+  L.code_off = take(sizeof(uint16_t) * (size_t)nnz);
   L.nslot_off = take(sizeof(int32_t) * (size_t)L.units);
   L.count_off = take(sizeof(int32_t) * (size_t)L.eb.ntiles);
   L.flag_off = take((size_t)L.eb.ntiles);
@@ -252,13 +253,16 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
       break;
     }
     case SEGTACO_BD:
-      // 32-row blocks on the matrix cores; lanes and groups are fixed
+      // blocks of 32 rows, or 64 (two MFMA row tiles sharing each staged B
+      // chunk: 7 % less B traffic on the banded 1M^2 BF16 workload at
+      // N = 256, but fewer workgroups per CU and 590 us per call against 500),
+      // on the matrix cores; lanes and groups are fixed
       if (b_dtype == SEGTACO_DT_F32)
         return set_error(SEGTACO_ERR_UNSUPPORTED, "bd: BF16 / FP16 storage only");
       if (plan->tile == 0) plan->tile = kBdRows;
-      if (plan->tile != kBdRows)
-        return set_error(SEGTACO_ERR_INVALID, "bd: tile (rows per block) must be %d",
-                         kBdRows);
+      if (plan->tile != kBdRows && plan->tile != 2 * kBdRows)
+        return set_error(SEGTACO_ERR_INVALID, "bd: tile (rows per block) must be %d or %d",
+                         kBdRows, 2 * kBdRows);
       if (plan->col_lanes == 0) plan->col_lanes = 1;
       if (plan->group == 0) plan->group = 1;
       if (plan->col_lanes != 1 || plan->group != 1)
@@ -504,7 +508,7 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     }
     case SEGTACO_BD: {
       if (!run) return SEGTACO_OK;  // nothing depends on A alone
-      a.nblocks = (int32_t)ceil_div(A->M, (int64_t)kBdRows);
+      a.nblocks = (int32_t)ceil_div(A->M, (int64_t)plan.tile);
       a.panels = (int32_t)ceil_div(N, (int64_t)bd_col_tiles(N) * kWavesPerBlock * 32);
       SEGTACO_HIP(mark(0));
       SEGTACO_HIP(mark(1));
@@ -558,7 +562,8 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     c.carry_tail = a.carry_tail;
     c.s_rec = static_cast<void *>(w + L.rec_off);
     c.slot_rows = reinterpret_cast<int32_t *>(w + L.rows_off);
-    c.walk_rows = reinterpret_cast<int32_t *>(w + L.walk_off);
+    c.int_rows = reinterpret_cast<int32_t *>(w + L.int_off);
+    c.codes = reinterpret_cast<uint16_t *>(w + L.code_off);
     c.n_slots = reinterpret_cast<int32_t *>(w + L.nslot_off);
     c.tile_count = reinterpret_cast<int32_t *>(w + L.count_off);
     c.tile_flags = reinterpret_cast<uint8_t *>(w + L.flag_off);

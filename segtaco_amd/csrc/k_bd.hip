@@ -4,16 +4,18 @@
 // The reference's schedules are all gather loops on vector ALUs (K1..K18);
 // for a banded or block-structured A that leaves the VALU as the bound (per
 // element one 16-bit unpack and one FMA). Here the rows are cut into blocks
-// of 32. For a block whose entries span a column window of at most 256
-// (cmin .. cmax), the block is scattered into a dense 32 x W tile in LDS
-// (zeros elsewhere) and multiplied by rows cmin .. cmin+W of B with
-// v_mfma_f32_32x32x8{bf16_1k,f16}: B is staged in LDS kBdKC rows at a time
-// as it lies in memory (each lane gathers its four K-consecutive operands
-// with 2-byte reads), the next chunk's loads in flight during this chunk's
-// MFMAs.
-// Each of the 4 waves owns CT tiles of 32 columns; accumulation is in
-// FP32 and every 16-bit product is exact in FP32. A block whose window is
-// wider falls back to a per-row walk on the vector ALUs (Neumaier
+// of 32 * RB (RB = 1 or 2 row tiles of 32). For a block whose entries span a
+// column window cmin .. cmax narrow enough for the LDS tile, the block is
+// scattered into a dense (32 RB) x W tile in LDS (zeros elsewhere) and
+// multiplied by rows cmin .. cmin+W of B with v_mfma_f32_32x32x16_{bf16,f16}:
+// B is staged in LDS kBdKC rows at a time as it lies in memory (each lane
+// gathers its eight K-consecutive operands with 2-byte reads), the next
+// chunk's loads in flight during this chunk's MFMAs; the RB row tiles share
+// every staged chunk, so B is read once per 32 RB rows. A 64-row block whose
+// window is too wide runs as two 32-row blocks.
+// Each of the 4 waves owns CT tiles of 32 columns per row tile; accumulation
+// is in FP32 and every 16-bit product is exact in FP32. A block whose window
+// is wider still falls back to a per-row walk on the vector ALUs (Neumaier
 // compensated for rows longer than comp_threshold). Every C element of a
 // block is written once (rows without entries as zeros): no clearing, no
 // atomics, deterministic.
@@ -25,27 +27,44 @@
 namespace segtaco {
 namespace {
 
-// LDS A-tile row stride (elements): 130 words, so the 32 rows that one MFMA
-// operand read touches start in 16 different bank pairs
-constexpr int kBdAStride = kBdMaxW + 4;
-// B rows staged per chunk (banded 1M^2, N = 256, BF16: 16 rows 530 us per
-// call, 32 rows 587 us, 64 rows 667 us: a smaller chunk fits more workgroups)
+// LDS A-tile row strides (elements), multiples of 8 so that each lane's
+// eight K-consecutive A operands are one aligned 16-byte read: 32-row tiles
+// up to kBdMaxW columns wide (264: rows 4 banks apart, so the 16 rows of one
+// read pass start on distinct 16-byte bank groups), 64-row tiles up to
+// kBdMaxW2 wide (200: rows 36 banks apart, likewise distinct). One buffer
+// holds either.
+constexpr int32_t kBdStride1 = kBdMaxW + 8;
+constexpr int32_t kBdStride2 = kBdMaxW2 + 8;
+template <int RB>
+constexpr int32_t bd_a_elems() {
+  return RB == 1 ? kBdRows * kBdStride1
+                 : (kBdRows * kBdStride1 > 2 * kBdRows * kBdStride2 ? kBdRows * kBdStride1
+                                                                   : 2 * kBdRows * kBdStride2);
+}
+// B rows staged per chunk: one MFMA K step (banded 1M^2, N = 256, BF16 with
+// 32-row blocks: 16 rows 530 us per call, 32 rows 587 us, 64 rows 667 us: a
+// smaller chunk fits more workgroups)
 constexpr int kBdKC = 16;
 // Scatter: entries per thread per round, and per round
 constexpr int kBdScat = 8;
 constexpr int32_t kBdRound = kBlockThreads * kBdScat;
 
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef _Float16 v4h __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
+// C[32x32] += A[32x16] B[16x32]: lane l holds A[l % 32][8 (l / 32) .. + 7]
+// and B[8 (l / 32) .. + 7][l % 32]; C[i][j] comes back in lane j + 32 ((i / 4)
+// % 2), register (i % 4) + 4 (i / 8).
 template <int DT>
-__device__ __forceinline__ v16f mfma32(v4s a, v4s b, v16f c) {
+__device__ __forceinline__ v16f mfma32(v8s a, v8s b, v16f c) {
   if constexpr (DT == 1)
-    return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a),
+                                                   __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
   else
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(__builtin_bit_cast(v4h, a),
-                                                __builtin_bit_cast(v4h, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, a),
+                                                  __builtin_bit_cast(v8h, b), c, 0, 0, 0);
 }
 
 // One stored value as its 16-bit encoding (1.0 when A has no values).
@@ -148,83 +167,55 @@ __device__ __forceinline__ void bd_store_chunk(uint16_t *sB, const BChunk<CT> &c
   }
 }
 
-// The MFMA B operand of one lane: B[k .. k+3][jc] from the row-major chunk.
-__device__ __forceinline__ v4s bd_b_operand(const uint16_t *sB, int32_t cols, int32_t k,
+// The MFMA B operand of one lane: B[k .. k+7][jc] from the row-major chunk.
+__device__ __forceinline__ v8s bd_b_operand(const uint16_t *sB, int32_t cols, int32_t k,
                                             int32_t jc) {
-  v4s b;
+  v8s b;
 #pragma unroll
-  for (int e = 0; e < 4; ++e) b[e] = (short)sB[(k + e) * cols + jc];
+  for (int e = 0; e < 8; ++e) b[e] = (short)sB[(k + e) * cols + jc];
   return b;
 }
 
-template <int DT, int CT>
-__global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
-  __shared__ __align__(16) uint16_t sA[kBdRows * kBdAStride];
-  __shared__ __align__(16) uint16_t sB[kBdKC * kWavesPerBlock * CT * 32];
-  __shared__ int32_t s_pos[kBdRows + 1];
-  __shared__ int32_t s_win[2];
-  __shared__ int32_t s_edge[kBdScat][kWavesPerBlock];  // each wave's last column
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int32_t kCols = BChunk<CT>::kCols;  // panel width
-  // XCD-contiguous order: workgroup b runs on XCD b % 8, and XCD x takes the
-  // units x*per .. x*per + per - 1 in ascending order, so the blocks resident
-  // on one XCD at a time are neighbours whose B windows overlap (a banded
-  // block's 96-row window shares 64 rows with each neighbour) and its L2
-  // serves the overlap. Round-robin order put neighbours on different XCDs,
-  // and every block fetched its whole window from memory.
-  const int32_t units = a.nblocks * a.panels;
-  const int32_t per = (units + 7) / 8;
-  const int32_t u = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
-  if (u >= units) return;  // padding of the last round (whole workgroup)
-  const int32_t rb = u / a.panels, panel = u % a.panels;
-  const int32_t r0 = rb * kBdRows;
-  const int32_t rows = min(kBdRows, a.M - r0);
-  const int32_t col0 = panel * kCols;
+// LDS of one workgroup.
+template <int CT, int RB>
+struct BdShared {
+  __align__(16) uint16_t A[bd_a_elems<RB>()];
+  __align__(16) uint16_t B[kBdKC * kWavesPerBlock * CT * 32];
+  int32_t pos[kBdRows * RB + 1];
+  int32_t win[2 * RB];                      // per 32-row half: first, last column
+  int32_t edge[kBdScat][kWavesPerBlock];    // each wave's last column per round
+};
 
-  // 1. the block's row pointer, and its column window (rows are sorted:
-  // their first and last entries bound it)
-  if (wave == 0) {
-    if (lane <= rows) s_pos[lane] = a.pos[r0 + lane];
-    int32_t lo = INT_MAX, hi = -1;
-    if (lane < rows) {
-      const int32_t s = a.pos[r0 + lane], e = a.pos[r0 + lane + 1];
-      if (s < e) {
-        lo = a.crd[s];
-        hi = a.crd[e - 1];
-      }
-    }
-    for (int d = 32; d >= 1; d >>= 1) {
-      lo = min(lo, __shfl_xor(lo, d, 64));
-      hi = max(hi, __shfl_xor(hi, d, 64));
-    }
-    if (lane == 0) {
-      s_win[0] = lo;
-      s_win[1] = hi;
-    }
-  }
-  __syncthreads();
-  const int32_t cmin = s_win[0], cmax = s_win[1];
-  if (cmax < 0 || cmax - cmin + 1 > kBdMaxW) {
-    // no entries (every row written as zeros) or too wide for the tile
-    bd_rows_valu<DT>(a, r0, rows, col0, kCols);
-    return;
-  }
+// The MFMA path for the RT row tiles r0 .. r0 + rows - 1 (rows <= 32 RT;
+// pos = their row pointer in LDS, rows + 1 entries) over the column window
+// [cmin, cmax] (at most stride - 8 wide), panel columns col0 ..; writes C
+// and returns true, or returns false without writing when the scatter finds
+// an entry outside the window or a row whose columns do not increase.
+// Called by the whole workgroup (uniform arguments).
+template <int DT, int CT, int RB, int RT>
+__device__ bool bd_mfma(const KArgs &a, BdShared<CT, RB> &sh, const int32_t *pos, int32_t r0,
+                        int32_t rows, int32_t col0, int32_t cmin, int32_t cmax,
+                        int32_t stride) {
+  constexpr int32_t kCols = BChunk<CT>::kCols;  // panel width
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int32_t W = cmax - cmin + 1;
-  const int32_t Wk = (W + 7) & ~7;  // MFMA K steps of 8
+  const int32_t Wk = (W + kBdKC - 1) & ~(kBdKC - 1);  // MFMA K steps of 16
   // branch-free 16-byte B loads (bd_load_chunk)
   const bool full8 = (reinterpret_cast<uintptr_t>(a.B) & 15) == 0 && (a.ldb & 7) == 0 &&
                      (a.N & 7) == 0;
+  uint16_t *sA = sh.A;
+  __syncthreads();  // a previous pass has finished with the LDS tiles
 
-  // 2. the first B chunk and the block's entries in flight together, while
-  // the A tile's window is zeroed
+  // the first B chunk and the block's entries in flight together, while the
+  // A tile's window is zeroed
   BChunk<CT> ch;
   bd_load_chunk<CT>(a, cmin, W, 0, col0, full8, ch);
-  const int32_t pb = s_pos[0], n = s_pos[rows] - pb;
-  for (int32_t i = wave; i < kBdRows; i += kWavesPerBlock)
+  const int32_t pb = pos[0], n = pos[rows] - pb;
+  for (int32_t i = wave; i < kBdRows * RT; i += kWavesPerBlock)
     for (int32_t w = lane; w < Wk / 2; w += 64)
-      reinterpret_cast<uint32_t *>(&sA[i * kBdAStride])[w] = 0u;
+      reinterpret_cast<uint32_t *>(&sA[i * stride])[w] = 0u;
   // Scatter, checking the column order it relies on as it goes: the window
-  // above and one tile cell per entry need every row's columns strictly
+  // and one tile cell per entry need every row's columns strictly
   // increasing. An entry outside the window is not written, and a block with
   // such an entry, or with a column that does not exceed its predecessor in
   // the same row (unsorted or duplicate columns), takes the per-row walk,
@@ -245,7 +236,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
     __syncthreads();  // tile zeroed (first round) / previous round's edges read
     if (lane == 63) {
 #pragma unroll
-      for (int u = 0; u < kBdScat; ++u) s_edge[u][wave] = cc[u];
+      for (int u = 0; u < kBdScat; ++u) sh.edge[u][wave] = cc[u];
     }
     __syncthreads();
 #pragma unroll
@@ -254,10 +245,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
       const int32_t up = __shfl_up(cc[u], 1, 64);
       if (j < n) {
         const int32_t p = pb + j;
-        int32_t lo = 0, hi = rows - 1;  // row: last i with s_pos[i] <= p
+        int32_t lo = 0, hi = rows - 1;  // row: last i with pos[i] <= p
         while (lo < hi) {
           const int32_t mid = (lo + hi + 1) >> 1;
-          if (s_pos[mid] <= p)
+          if (pos[mid] <= p)
             lo = mid;
           else
             hi = mid - 1;
@@ -265,61 +256,128 @@ __global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
         const int32_t c = cc[u];
         const bool in = c >= cmin && c <= cmax;
         const int32_t prev = lane > 0  ? up
-                             : wave > 0 ? s_edge[u][wave - 1]
-                             : u > 0    ? s_edge[u - 1][kWavesPerBlock - 1]
+                             : wave > 0 ? sh.edge[u][wave - 1]
+                             : u > 0    ? sh.edge[u - 1][kWavesPerBlock - 1]
                                         : prev_last;
-        bad |= (!in || (p != s_pos[lo] && prev >= c)) ? 1 : 0;
-        if (in) sA[lo * kBdAStride + (c - cmin)] = vv[u];
+        bad |= (!in || (p != pos[lo] && prev >= c)) ? 1 : 0;
+        if (in) sA[lo * stride + (c - cmin)] = vv[u];
       }
     }
     // a round that is not the last is full: its last entry is thread
     // kBlockThreads - 1's last
-    prev_last = s_edge[kBdScat - 1][kWavesPerBlock - 1];
+    prev_last = sh.edge[kBdScat - 1][kWavesPerBlock - 1];
   }
-  if (__syncthreads_or(bad)) {
-    bd_rows_valu<DT>(a, r0, rows, col0, kCols);
-    return;
-  }
+  if (__syncthreads_or(bad)) return false;
 
-  // 3. K loop over the window, kBdKC rows of B at a time: a chunk's
-  // registers go to LDS and are refilled at once with the next chunk, whose
-  // loads are in flight during this chunk's MFMAs. (Three chunks in flight,
-  // each refilled with the chunk three steps ahead, measured slower on the
-  // banded 1M^2 BF16 workload at N = 256: 564 us vs 505 us per call.)
-  v16f acc[CT];
+  // K loop over the window, kBdKC rows of B at a time: a chunk's registers go
+  // to LDS and are refilled at once with the next chunk, whose loads are in
+  // flight during this chunk's MFMAs. Measured on the banded 1M^2 BF16
+  // workload at N = 256 with 32-row blocks, 500 us per call; three chunks in
+  // flight, each refilled with the chunk three steps ahead: 564 us; two LDS
+  // buffers with one barrier per chunk instead of two: 524 us (more LDS per
+  // workgroup, fewer workgroups per CU).
+  v16f acc[RT][CT];
 #pragma unroll
-  for (int ct = 0; ct < CT; ++ct)
+  for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[ct][r] = 0.0f;
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[rt][ct][r] = 0.0f;
+  const int32_t kl = 8 * (lane >> 5);  // this lane's K offset in a step
   for (int32_t k0 = 0; k0 < Wk; k0 += kBdKC) {
     __syncthreads();  // A scattered / previous chunk consumed
-    bd_store_chunk<CT>(sB, ch);
+    bd_store_chunk<CT>(sh.B, ch);
     if (k0 + kBdKC < Wk) bd_load_chunk<CT>(a, cmin, W, k0 + kBdKC, col0, full8, ch);
     __syncthreads();
-    const int32_t ksteps = min(kBdKC, Wk - k0) / 8;
-    for (int32_t ks = 0; ks < ksteps; ++ks) {
-      const int32_t kk = ks * 8 + 4 * (lane >> 5);
-      const v4s av =
-          *reinterpret_cast<const v4s *>(&sA[(lane & 31) * kBdAStride + k0 + kk]);
+    v8s bv[CT];
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int32_t jc = (wave * CT + ct) * 32 + (lane & 31);
-        const v4s bv = bd_b_operand(sB, kCols, kk, jc);
-        acc[ct] = mfma32<DT>(av, bv, acc[ct]);
-      }
+    for (int ct = 0; ct < CT; ++ct)
+      bv[ct] = bd_b_operand(sh.B, kCols, kl, (wave * CT + ct) * 32 + (lane & 31));
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const v8s av =
+          *reinterpret_cast<const v8s *>(&sA[(rt * 32 + (lane & 31)) * stride + k0 + kl]);
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = mfma32<DT>(av, bv[ct], acc[rt][ct]);
     }
   }
 
-  // 4. C tile: lane holds rows (r%4) + 4*(lane/32) + 8*(r/4), column lane%32
+  // C tiles: lane holds rows (r%4) + 4*(lane/32) + 8*(r/4), column lane%32
 #pragma unroll
   for (int ct = 0; ct < CT; ++ct) {
     const int32_t col = col0 + (wave * CT + ct) * 32 + (lane & 31);
     if (col >= a.N) continue;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int32_t i = (r & 3) + 4 * (lane >> 5) + 8 * (r >> 2);
-      if (i < rows) a.C[(int64_t)(r0 + i) * a.ldc + col] = acc[ct][r];
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int32_t i = rt * 32 + (r & 3) + 4 * (lane >> 5) + 8 * (r >> 2);
+        if (i < rows) a.C[(int64_t)(r0 + i) * a.ldc + col] = acc[rt][ct][r];
+      }
+  }
+  return true;
+}
+
+template <int DT, int CT, int RB>
+__global__ __launch_bounds__(kBlockThreads) void k_bd(const KArgs a) {
+  __shared__ BdShared<CT, RB> sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int32_t kCols = BChunk<CT>::kCols;  // panel width
+  constexpr int32_t kRowsB = kBdRows * RB;      // rows per block
+  // XCD-contiguous order: workgroup b runs on XCD b % 8, and XCD x takes the
+  // units x*per .. x*per + per - 1 in ascending order, so the blocks resident
+  // on one XCD at a time are neighbours whose B windows overlap (a banded
+  // block's window shares 64 rows with each neighbour) and its L2 serves the
+  // overlap. Round-robin order put neighbours on different XCDs, and every
+  // block fetched its whole window from memory.
+  const int32_t units = a.nblocks * a.panels;
+  const int32_t per = (units + 7) / 8;
+  const int32_t u = (int32_t)(blockIdx.x & 7) * per + (int32_t)(blockIdx.x >> 3);
+  if (u >= units) return;  // padding of the last round (whole workgroup)
+  const int32_t rb = u / a.panels, panel = u % a.panels;
+  const int32_t r0 = rb * kRowsB;
+  const int32_t rows = min(kRowsB, a.M - r0);
+  const int32_t col0 = panel * kCols;
+
+  // 1. the block's row pointer, and each 32-row half's column window (rows
+  // are sorted: their first and last entries bound it)
+  if (wave == 0) {
+    int32_t lo = INT_MAX, hi = -1;
+    if (lane < rows) {
+      const int32_t s = a.pos[r0 + lane], e = a.pos[r0 + lane + 1];
+      sh.pos[lane] = s;
+      if (lane == rows - 1) sh.pos[rows] = e;
+      if (s < e) {
+        lo = a.crd[s];
+        hi = a.crd[e - 1];
+      }
     }
+    for (int d = 16; d >= 1; d >>= 1) {
+      lo = min(lo, __shfl_xor(lo, d, 64));
+      hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    if ((lane & 31) == 0 && (lane >> 5) < RB) {
+      sh.win[2 * (lane >> 5)] = lo;
+      sh.win[2 * (lane >> 5) + 1] = hi;
+    }
+  }
+  __syncthreads();
+  if constexpr (RB == 2) {
+    const int32_t lo = min(sh.win[0], sh.win[2]), hi = max(sh.win[1], sh.win[3]);
+    if (hi >= 0 && hi - lo + 1 <= kBdMaxW2) {
+      if (!bd_mfma<DT, CT, RB, 2>(a, sh, sh.pos, r0, rows, col0, lo, hi, kBdStride2))
+        bd_rows_valu<DT>(a, r0, rows, col0, kCols);
+      return;
+    }
+  }
+  // 32-row halves, each on its own window
+  for (int32_t h = 0; h < RB && 32 * h < rows; ++h) {
+    const int32_t hr0 = r0 + 32 * h, hrows = min(32, rows - 32 * h);
+    const int32_t lo = sh.win[2 * h], hi = sh.win[2 * h + 1];
+    // no entries (every row written as zeros) or too wide for the tile
+    if (hi < 0 || hi - lo + 1 > kBdMaxW ||
+        !bd_mfma<DT, CT, RB, 1>(a, sh, sh.pos + 32 * h, hr0, hrows, col0, lo, hi, kBdStride1))
+      bd_rows_valu<DT>(a, hr0, hrows, col0, kCols);
   }
 }
 
@@ -335,11 +393,16 @@ hipError_t launch_bd(int dt, const KArgs &a, hipStream_t s) {
   if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
   const dim3 g((unsigned)grid), b(kBlockThreads);
   const int ct = bd_col_tiles(a.N);
-  switch (dt * 4 + ct) {
-    case 1 * 4 + 1: hipLaunchKernelGGL((k_bd<1, 1>), g, b, 0, s, a); break;
-    case 1 * 4 + 2: hipLaunchKernelGGL((k_bd<1, 2>), g, b, 0, s, a); break;
-    case 2 * 4 + 1: hipLaunchKernelGGL((k_bd<2, 1>), g, b, 0, s, a); break;
-    case 2 * 4 + 2: hipLaunchKernelGGL((k_bd<2, 2>), g, b, 0, s, a); break;
+  const int rb = a.tile / kBdRows;
+  switch (dt * 16 + ct * 4 + rb) {
+    case 1 * 16 + 1 * 4 + 1: hipLaunchKernelGGL((k_bd<1, 1, 1>), g, b, 0, s, a); break;
+    case 1 * 16 + 2 * 4 + 1: hipLaunchKernelGGL((k_bd<1, 2, 1>), g, b, 0, s, a); break;
+    case 2 * 16 + 1 * 4 + 1: hipLaunchKernelGGL((k_bd<2, 1, 1>), g, b, 0, s, a); break;
+    case 2 * 16 + 2 * 4 + 1: hipLaunchKernelGGL((k_bd<2, 2, 1>), g, b, 0, s, a); break;
+    case 1 * 16 + 1 * 4 + 2: hipLaunchKernelGGL((k_bd<1, 1, 2>), g, b, 0, s, a); break;
+    case 1 * 16 + 2 * 4 + 2: hipLaunchKernelGGL((k_bd<1, 2, 2>), g, b, 0, s, a); break;
+    case 2 * 16 + 1 * 4 + 2: hipLaunchKernelGGL((k_bd<2, 1, 2>), g, b, 0, s, a); break;
+    case 2 * 16 + 2 * 4 + 2: hipLaunchKernelGGL((k_bd<2, 2, 2>), g, b, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

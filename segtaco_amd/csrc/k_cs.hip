@@ -15,16 +15,17 @@
 //     first row of each, k_tile_rows_fix); a unit is G consecutive tiles,
 //     one per lane group of one 512-thread workgroup, eight per wave;
 //   * a preparation pass, k_cs_count then k_cs_sort (one wave per tile),
-//     decides per wave of the unit whether its tiles sweep: they do when
-//     their interior rows (rows that start and end inside a tile) fit the
-//     LDS slots beside those of the unit's earlier sweeping waves. A
-//     sweeping tile's rows get LDS slots — slot 2g for the row tile g
-//     continues from an earlier tile (head), 2g + 1 for the row it leaves
-//     open for a later one (tail), 2G + k for the unit's k-th interior row —
-//     and its entries are written sorted by (column, position) as one record
-//     each (column, slot, value). A walking tile's entries stay in CSR order
-//     and name their row by its place in the tile (head, tail, k-th interior
-//     row, whose row id goes to a side array);
+//     names each entry's row by its place in the tile (head, tail, k-th
+//     interior row, i.e. a row that starts and ends inside the tile, whose
+//     row id goes to a side array) and decides per wave of the unit whether
+//     its tiles sweep: they do when their interior rows fit the LDS slots
+//     beside those of the unit's earlier sweeping waves. A sweeping tile's
+//     rows get LDS slots — slot 2g for the row tile g continues from an
+//     earlier tile (head), 2g + 1 for the row it leaves open for a later one
+//     (tail), from 2G on the unit's interior rows — and its entries are
+//     written sorted by (column, position) as one record each (column, slot,
+//     value). A walking tile's entries stay in CSR order with the in-tile
+//     names;
 //   * k_eb_cs, per unit and per column panel of W = 8*V columns: zero the
 //     unit's LDS slots, let every sweeping lane group run its sorted tile —
 //     per entry one 16-byte gather per lane and one read-modify-write of its
@@ -160,8 +161,8 @@ __device__ __forceinline__ bool cs_starts_interior(const CsArgs &a, const PrepTi
 // its tiles fit the slots left after the 2G head / tail slots and the
 // earlier sweeping waves' rows, and walks otherwise (a later, smaller wave
 // may still sweep). Returns tile t's first interior slot, or -1 when its
-// wave walks; n_slots = slots the unit uses.
-__device__ int32_t cs_slot_base(const CsArgs &a, int32_t t, int32_t &n_slots) {
+// wave walks; n_slots = slots the unit uses, count = tile t's interior rows.
+__device__ int32_t cs_slot_base(const CsArgs &a, int32_t t, int32_t &n_slots, int32_t &count) {
   const int lane = threadIdx.x & 63;
   const int32_t t0 = (t / a.group) * a.group;
   const int32_t g = t - t0;
@@ -181,6 +182,7 @@ __device__ int32_t cs_slot_base(const CsArgs &a, int32_t t, int32_t &n_slots) {
     }
   }
   n_slots = 2 * a.group + used;
+  count = __shfl(cnt, g, 64);
   return base;
 }
 
@@ -221,12 +223,14 @@ __device__ __forceinline__ void wave_bitonic(uint32_t (&key)[R]) {
 }
 
 // Prep, one wave per tile (many waves per CU hide the latency of the
-// per-entry row searches). k_cs_count: the tile's interior rows and its
-// head / tail flags. k_cs_sort: the unit's slot plan (cs_slot_base), the
-// code of every entry's row — LDS slot when sweeping; 0 head, 1 tail, 2 + k
-// for the k-th interior row when walking — and the rows behind the codes,
-// then the records: sweeping tiles sorted by (column, position), walking
-// tiles in CSR order.
+// per-entry row searches). k_cs_count: per tile its head / tail flags, its
+// interior rows (count, and the row of the k-th in int_rows) and per entry
+// the tile-local code of its row (0 head, 1 tail, 2 + k the k-th interior
+// row). k_cs_sort: the unit's slot plan (cs_slot_base), then the records:
+// walking tiles in CSR order with the tile-local codes, sweeping tiles
+// sorted by (column, position) with the codes mapped to LDS slots, and the
+// rows of the interior slots. Every load k_cs_sort makes is issued before
+// anything waits, so the kernel costs one memory round trip.
 template <int E>
 __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
   __shared__ int32_t s_pos[kCsPrepWaves][kCsMaxTile + kCsPosSlack];
@@ -237,56 +241,17 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
   int32_t *sp = s_pos[wave];
   PrepTile<E> P;
   cs_tile_rows<E>(a, t, sp, P);
-  int32_t n = 0;
-#pragma unroll
-  for (int k = 0; k < E; ++k) n += cs_starts_interior<E>(a, P, sp, k) ? 1 : 0;
-  n = wave_sum(n);
-  if (lane == 0) {
-    a.tile_count[t] = n;
-    a.tile_flags[t] = (uint8_t)((P.head ? 1 : 0) | (P.tail ? 2 : 0));
-  }
-}
-
-template <int E>
-__global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
-  constexpr int R = E <= 1 ? 1 : E <= 2 ? 2 : E <= 4 ? 4 : 8;  // sort registers
-  __shared__ int32_t s_pos[kCsPrepWaves][kCsMaxTile + kCsPosSlack];
-  __shared__ int32_t s_code[kCsPrepWaves][kCsMaxTile];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int32_t t = blockIdx.x * kCsPrepWaves + wave;
-  if (t >= a.ntiles) return;  // uniform over the wave
-  int32_t *sp = s_pos[wave];
-  int32_t *sc = s_code[wave];
-  const int32_t unit = t / a.group;
-  const int32_t g = t - unit * a.group;
-  int32_t n_slots = 0;
-  const int32_t base = cs_slot_base(a, t, n_slots);
-  const bool walk = base < 0;
-  if (g == 0 && lane == 0) a.n_slots[unit] = n_slots;
-
-  PrepTile<E> P;
-  cs_tile_rows<E>(a, t, sp, P);
-  if (walk && lane == 0) a.tile_flags[t] |= 4;
-  const int32_t *__restrict__ crd = a.crd;
-  int32_t code[E], col[E];
+  int32_t code[E];
   int32_t running = 0;  // interior rows started by earlier chunks
 #pragma unroll
   for (int k = 0; k < E; ++k) {
-    const int32_t i = lane + 64 * k;
     const bool st = cs_starts_interior<E>(a, P, sp, k);
     const int32_t inc = wave_incl_scan(st ? 1 : 0, lane) + running;
     running = __shfl(inc, 63, 64);
     // the entry that starts an interior row names the row's place k in the
     // tile; the row's other entries take it from the running maximum below
     code[k] = st ? inc - 1 : -1;
-    if (st) {
-      if (walk)
-        a.walk_rows[(int64_t)t * a.tile + inc - 1] = P.row[k];
-      else
-        a.slot_rows[(int64_t)unit * a.cap + base + inc - 1] = P.row[k];
-    }
-    col[k] = i < P.cn ? crd[P.s + i] : INT_MAX;
+    if (st) a.int_rows[(int64_t)t * a.tile + inc - 1] = P.row[k];
   }
   int32_t carry = -1;
 #pragma unroll
@@ -298,48 +263,89 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
     }
     m = max(m, carry);
     carry = __shfl(m, 63, 64);
+    const int32_t i = lane + 64 * k;
     const int32_t r = P.row[k];
     const bool hd = P.head && r == P.r_first;
     const bool tl = !hd && P.tail && r == P.r_last;
-    if (walk)
-      code[k] = hd ? 0 : tl ? 1 : 2 + m;
-    else
-      code[k] = hd ? 2 * g : tl ? 2 * g + 1 : base + m;
+    if (i < P.cn) a.codes[P.s + i] = (uint16_t)(hd ? 0 : tl ? 1 : 2 + m);
   }
-  const uint32_t one = a.one_bits;  // A without values reads 1.0
-  // record of entry i (its place in the tile) at sorted place d
-  auto put = [&](int32_t d, int32_t i, int32_t c, int32_t cd) {
-    uint32_t vb = one;
+  if (lane == 0) {
+    a.tile_count[t] = running;
+    a.tile_flags[t] = (uint8_t)((P.head ? 1 : 0) | (P.tail ? 2 : 0));
+  }
+}
+
+template <int E>
+__global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
+  constexpr int R = E <= 1 ? 1 : E <= 2 ? 2 : E <= 4 ? 4 : 8;  // sort registers
+  __shared__ int32_t s_code[kCsPrepWaves][kCsMaxTile];
+  __shared__ uint32_t s_val[kCsPrepWaves][kCsMaxTile];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int32_t t = blockIdx.x * kCsPrepWaves + wave;
+  if (t >= a.ntiles) return;  // uniform over the wave
+  const int32_t unit = t / a.group;
+  const int32_t g = t - unit * a.group;
+  const int32_t s = t * a.tile;
+  const int32_t cn = (int32_t)min((int64_t)a.nnz - s, (int64_t)a.tile);
+  // every load of the tile at once: codes, columns, values, interior rows
+  int32_t cd[E], col[E], irow[E];
+  uint32_t vb[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int32_t i = min(lane + 64 * k, cn - 1);
+    cd[k] = a.codes[s + i];
+    col[k] = a.crd[s + i];
+    vb[k] = a.one_bits;  // A without values reads 1.0
     if (a.vals)
-      vb = a.elem == 4 ? static_cast<const uint32_t *>(a.vals)[P.s + i]
-                       : (uint32_t) static_cast<const uint16_t *>(a.vals)[P.s + i];
-    const int64_t dst = (int64_t)P.s + d;
+      vb[k] = a.elem == 4 ? static_cast<const uint32_t *>(a.vals)[s + i]
+                          : (uint32_t) static_cast<const uint16_t *>(a.vals)[s + i];
+    irow[k] = a.int_rows[(int64_t)t * a.tile + i];
+  }
+  int32_t n_slots = 0, count = 0;
+  const int32_t base = cs_slot_base(a, t, n_slots, count);
+  const bool walk = base < 0;
+  if (g == 0 && lane == 0) a.n_slots[unit] = n_slots;
+  auto put = [&](int32_t d, int32_t c, int32_t code, uint32_t v) {
+    const int64_t dst = (int64_t)s + d;
     if (a.packed)
       static_cast<uint2 *>(a.s_rec)[dst] =
-          make_uint2(((uint32_t)c << kCsCodeBits) | (uint32_t)cd, vb);
+          make_uint2(((uint32_t)c << kCsCodeBits) | (uint32_t)code, v);
     else
-      static_cast<CsWide *>(a.s_rec)[dst] = CsWide{(uint32_t)c, (uint32_t)cd, vb};
+      static_cast<CsWide *>(a.s_rec)[dst] = CsWide{(uint32_t)c, (uint32_t)code, v};
   };
   if (walk) {
+    if (lane == 0) a.tile_flags[t] |= 4;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int32_t i = lane + 64 * k;
-      if (i < P.cn) put(i, i, col[k], code[k]);
+      if (i < cn) put(i, col[k], cd[k], vb[k]);
     }
     return;
+  }
+  // sweeping: tile-local codes to LDS slots, and the rows of its interior
+  // slots (interior row i of the tile holds slot base + i)
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int32_t i = lane + 64 * k;
+    if (i < count) a.slot_rows[(int64_t)unit * a.cap + base + i] = irow[k];
+    cd[k] = cd[k] == 0 ? 2 * g : cd[k] == 1 ? 2 * g + 1 : base + cd[k] - 2;
   }
   if (a.packed) {
     // columns below 2^(32 - kCsCodeBits) <= 2^20: the key (column << 9 |
     // place) is unique, fits 29 bits and orders by (column, position);
     // places past the tile sort last
+    int32_t *sc = s_code[wave];
+    uint32_t *sv = s_val[wave];
     uint32_t key[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int32_t i = lane + 64 * r;
       key[r] = 0xffffffffu;
-      if (r < E && i < P.cn) {
+      if (r < E && i < cn) {
         key[r] = ((uint32_t)col[r] << kCsIdxBits) | (uint32_t)i;
-        sc[i] = code[r];
+        sc[i] = cd[r];
+        sv[i] = vb[r];
       }
     }
     wave_sync();
@@ -347,9 +353,9 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int32_t d = lane + 64 * r;
-      if (d < P.cn) {
+      if (d < cn) {
         const int32_t i = (int32_t)(key[r] & (kCsMaxTile - 1));
-        put(d, i, (int32_t)(key[r] >> kCsIdxBits), sc[i]);
+        put(d, (int32_t)(key[r] >> kCsIdxBits), sc[i], sv[i]);
       }
     }
     return;
@@ -360,8 +366,8 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
   for (int k = 0; k < E; ++k) rank[k] = 0;
 #pragma unroll
   for (int k2 = 0; k2 < E; ++k2) {
-    if (64 * k2 >= P.cn) break;  // uniform
-    const int32_t lim = min(64, P.cn - 64 * k2);
+    if (64 * k2 >= cn) break;  // uniform
+    const int32_t lim = min(64, cn - 64 * k2);
     for (int32_t l = 0; l < lim; ++l) {
       const int32_t cj = __builtin_amdgcn_readlane(col[k2], l);
       const int32_t j = 64 * k2 + l;
@@ -373,7 +379,7 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int32_t i = lane + 64 * k;
-    if (i < P.cn) put(rank[k], i, col[k], code[k]);
+    if (i < cn) put(rank[k], col[k], cd[k], vb[k]);
   }
 }
 
@@ -507,7 +513,7 @@ __device__ __forceinline__ void cs_sweep(const CsArgs &a, float *acc_lds, int32_
 
 // Walk of a tile in CSR order (its wave did not fit the LDS slots): a
 // register accumulator per row, stored when the row code changes — code 0
-// to the head carry, 1 to the tail carry, 2 + k to C's row walk_rows[k] of
+// to the head carry, 1 to the tail carry, 2 + k to C's row int_rows[k] of
 // the tile (loaded with the gathers, so it is in registers by then).
 template <int DT, int V, bool kPacked>
 __device__ __forceinline__ void cs_walk(const CsArgs &a, int32_t t, int32_t s, int32_t e,
@@ -520,7 +526,7 @@ __device__ __forceinline__ void cs_walk(const CsArgs &a, int32_t t, int32_t s, i
     int32_t row[kB];
   };
   const int32_t col_ld = colok ? col : 0;
-  const int32_t *rows = a.walk_rows + (int64_t)t * a.tile;
+  const int32_t *rows = a.int_rows + (int64_t)t * a.tile;
   auto gather = [&](const M &m, G &g) {
 #pragma unroll
     for (int u = 0; u < kB; ++u) {

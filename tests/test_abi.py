@@ -254,14 +254,16 @@ def test_eb_sr_default_tile_follows_product_size():
 
 
 def test_bd_plan_resolution():
-    """bd: 16-bit storage only, 32-row blocks, lanes and groups fixed at 1."""
+    """bd: 16-bit storage only, 32-row blocks by default (64 on request),
+    lanes and groups fixed at 1."""
     B16 = _desc(M=1000, K=1000, nnz=20000, dtype=S.DT_BF16)
     p = S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("bd"), 256, 256)
     assert (p.tile, p.col_lanes, p.group, p.comp_threshold) == (32, 1, 1, 1024)
     assert S.workspace_bytes(B16, 256, p) == 0
+    assert S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("bd", tile=64), 256, 256).tile == 64
     with pytest.raises(S.SegtacoError):  # FP32 has no faster matrix-core path here
         S.plan_resolve(_desc(M=1000, K=1000, nnz=20000), 64, S.DT_F32, S.make_plan("bd"))
-    for bad in (dict(tile=64), dict(col_lanes=4), dict(group=2)):
+    for bad in (dict(tile=128), dict(tile=48), dict(col_lanes=4), dict(group=2)):
         with pytest.raises(S.SegtacoError):
             S.plan_resolve(B16, 256, S.DT_BF16, S.make_plan("bd", **bad), 256, 256)
 

@@ -1,6 +1,7 @@
-"""GPU parity of the block-dense schedule (bd): BF16 / FP16 storage, 32-row
-blocks on the matrix cores when their columns span at most 256, a per-row
-vector walk otherwise.
+"""GPU parity of the block-dense schedule (bd): BF16 / FP16 storage, blocks
+of 32 rows (default) on the matrix cores when their columns span at most
+256; blocks of 64 rows (tile = 64) when they span at most 192, else each
+32-row half on its own; a per-row vector walk otherwise.
 
 Tolerances (as for every 16-bit path):
   * vs FP64 of the *rounded* inputs, oracle scaled metric: <= 1e-5;
@@ -56,6 +57,16 @@ def _case(name):
             spans[32 * b] = (c0, c0 + 1, 1.0)  # pin the window's first column
             spans[32 * b + 31] = (c0 + w - 1, c0 + w, 1.0)  # and its last
         return (*_rows(rng, M, K, spans), K)
+    if name == "window_edges64":  # 64-row blocks of width 192 (one tile) and 193 (halves)
+        M, K = 64 * 4 + 17, 1500
+        spans = []
+        for b, w in enumerate([192, 193, 150, 192, 60]):
+            c0 = 250 * b
+            for i in range(64):
+                spans.append((c0, c0 + w, 0.3))
+            spans[64 * b] = (c0, c0 + 1, 1.0)
+            spans[64 * b + 63] = (c0 + w - 1, c0 + w, 1.0)
+        return (*_rows(rng, M, K, spans[:M]), K)
     if name == "powerlaw_fallback":  # nothing dense: all blocks on the vector path
         pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 4, 3000, 3000)
         return pos, crd, vals, 3000
@@ -71,11 +82,11 @@ def _case(name):
     raise KeyError(name)
 
 
-CASES = ["banded_bw32", "banded_ragged", "window_edges", "powerlaw_fallback",
-         "long_rows_fallback", "near_k_edge"]
+CASES = ["banded_bw32", "banded_ragged", "window_edges", "window_edges64",
+         "powerlaw_fallback", "long_rows_fallback", "near_k_edge"]
 
 
-def _run(pos, crd, vals, B, dtype, ldb=None, ldc=None, runs=1):
+def _run(pos, crd, vals, B, dtype, ldb=None, ldc=None, runs=1, tile=0):
     K, N = B.shape
     A = DeviceCSR(pos, crd, vals, K, dtype=dtype)
     ldb = ldb or N
@@ -83,8 +94,8 @@ def _run(pos, crd, vals, B, dtype, ldb=None, ldc=None, runs=1):
     Bp[:, :N] = B
     outs = []
     for _ in range(runs):
-        C = run_spmm(A, dense_dev(Bp, dtype), N, S.make_plan("bd"), ldb=ldb, ldc=ldc,
-                     b_dtype=dtype)
+        C = run_spmm(A, dense_dev(Bp, dtype), N, S.make_plan("bd", tile=tile), ldb=ldb,
+                     ldc=ldc, b_dtype=dtype)
         outs.append(C[: A.M, :N].cpu().numpy())
     return outs
 
@@ -98,14 +109,15 @@ def _check(pos, crd, vals, B, C, dtype, what):
     assert fro <= 1e-2, f"{what}: vs FP64 of FP32 inputs {fro:.3e}"
 
 
+@pytest.mark.parametrize("tile", [64, 32])
 @pytest.mark.parametrize("N", [256, 128, 40, 7, 300])
 @pytest.mark.parametrize("dtype", ["bf16", "fp16"])
 @pytest.mark.parametrize("case", CASES)
-def test_bd_parity(case, dtype, N):
+def test_bd_parity(case, dtype, N, tile):
     pos, crd, vals, K = _case(case)
     B = np.random.default_rng(N).uniform(-1, 1, (K, N)).astype(np.float32)
-    print(f"[bd] {case} {dtype} N={N} M={len(pos)-1} nnz={len(crd)}", flush=True)
-    C, C2 = _run(pos, crd, vals, B, dtype, runs=2)
+    print(f"[bd] {case} {dtype} N={N} tile={tile} M={len(pos)-1} nnz={len(crd)}", flush=True)
+    C, C2 = _run(pos, crd, vals, B, dtype, runs=2, tile=tile)
     _check(pos, crd, vals, B, C, dtype, f"{case}/{dtype}/N={N}")
     assert np.array_equal(C.view(np.uint32), C2.view(np.uint32)), "not deterministic"
 
@@ -137,16 +149,17 @@ def _disorder(name):
     return pos, crd, vals, 32 * 12
 
 
+@pytest.mark.parametrize("tile", [64, 32])
 @pytest.mark.parametrize("N", [256, 64])
 @pytest.mark.parametrize("dtype", ["bf16", "fp16"])
-def test_bd_unsorted_and_duplicate_columns(dtype, N):
+def test_bd_unsorted_and_duplicate_columns(dtype, N, tile):
     """bd checks each block's column order before scattering it: blocks with
     duplicate, unsorted or out-of-window entries take the per-row walk and
     still give the product of the stored entries (duplicates summed)."""
     pos, crd, vals, K = _disorder("disorder")
     B = np.random.default_rng(N + 1).uniform(-1, 1, (K, N)).astype(np.float32)
-    print(f"[bd] disorder {dtype} N={N} nnz={len(crd)}", flush=True)
-    C, C2 = _run(pos, crd, vals, B, dtype, runs=2)
+    print(f"[bd] disorder {dtype} N={N} tile={tile} nnz={len(crd)}", flush=True)
+    C, C2 = _run(pos, crd, vals, B, dtype, runs=2, tile=tile)
     _check(pos, crd, vals, B, C, dtype, f"disorder/{dtype}/N={N}")
     assert np.array_equal(C.view(np.uint32), C2.view(np.uint32)), "not deterministic"
 
