@@ -298,10 +298,10 @@ def profiled_traffic(name, vec=4):
     path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
-    key = f"traffic_bytes_per_launch k_{name}<0, {vec}>:"
+    key = f"traffic_bytes_per_launch k_{name}<0, {vec}"  # then ">:" or ", <more>>:"
     with open(path) as f:
         for line in f:
-            if line.startswith(key):
+            if line.startswith(key) and line[len(key)] in ">,":
                 return int(line.split(":", 1)[1].strip())
     return None
 
