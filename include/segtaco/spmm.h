@@ -266,7 +266,9 @@ int32_t segtaco_spmm_prepare(const segtaco_csr_t *A, int32_t b_dtype, int64_t ld
 /* segtaco_spmm on a workspace whose last preparation (by
  * segtaco_spmm_prepare or by a segtaco_spmm call) was of this product: the
  * same device arrays of A with unchanged contents, the same N and storage
- * type, and a plan that resolves, for these operands, to the prepared one.
+ * type, and a plan that resolves, for these operands, to the prepared one;
+ * the workspace itself must not have been written since by anything else
+ * (the caller, a batch, another library call given the same memory).
  * Skips the preparation (eb_cs); every other step runs as in segtaco_spmm,
  * and C is bit-identical to it. A workspace prepared for another product is
  * refused (SEGTACO_ERR_INVALID) for eb_cs; whether A's contents changed is
