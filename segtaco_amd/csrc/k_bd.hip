@@ -45,8 +45,13 @@ constexpr int32_t bd_a_elems() {
 // 32-row blocks: 16 rows 530 us per call, 32 rows 587 us, 64 rows 667 us: a
 // smaller chunk fits more workgroups)
 constexpr int kBdKC = 16;
-// Scatter: entries per thread per round, and per round
-constexpr int kBdScat = 8;
+// Scatter: entries per thread per round, and per round. The scatter's
+// per-entry registers set the kernel's peak register count: six entries
+// instead of eight bring it from 53 to 47 VGPRs (plus 32 accumulators),
+// which lets 6 rather than 5 workgroups share a CU (banded 1M^2 BF16,
+// N = 256: 524-537 vs 561-565 us per call in one side-by-side run; four
+// entries: 528-543 us)
+constexpr int kBdScat = 6;
 constexpr int32_t kBdRound = kBlockThreads * kBdScat;
 
 typedef short v8s __attribute__((ext_vector_type(8)));
