@@ -44,6 +44,19 @@ __device__ __forceinline__ float load_val(const void *__restrict__ vals,
   }
 }
 
+
+// load_val for an entry stream read once (non-temporal: not kept in L2, which
+// the gathers of B can use instead).
+template <int DT>
+__device__ __forceinline__ float load_val_stream(const void *__restrict__ vals, int64_t p) {
+  if (vals == nullptr) return 1.0f;
+  if constexpr (DT == 0) {
+    return __builtin_nontemporal_load(static_cast<const float *>(vals) + p);
+  } else {
+    return half_to_f32<DT>(__builtin_nontemporal_load(static_cast<const uint16_t *>(vals) + p));
+  }
+}
+
 // V consecutive elements starting at element index i (i aligned to V).
 template <int DT, int V>
 __device__ __forceinline__ void load_vec(const void *__restrict__ base,
