@@ -341,6 +341,22 @@ int32_t segtaco_tile_rows(const int32_t *pos, int32_t M, int32_t tile, int32_t n
 int32_t segtaco_spmm_workspace_layout(const segtaco_csr_t *A, int32_t N,
                                       const segtaco_plan_t *plan, int64_t *out);
 
+/* Where an eb_cs call with this resolved plan keeps its preparation in the
+ * workspace (after the element-balanced part above): out[0] = units (tiles
+ * per unit: plan->group), out[1] = LDS slots per unit, out[2] = 1 for 64-bit
+ * records ({column << 12 | code, value bits}), 0 for 96-bit ones ({column,
+ * code, value bits}); byte offsets of out[3] the records (one per stored
+ * entry: a sweeping tile's sorted by column and position with its rows' LDS
+ * slots as codes, a walking tile's in CSR order with 0 head, 1 tail, 2 + k
+ * its k-th interior row), out[4] the rows of each unit's slots (int32 [units]
+ * [slots]), out[5] each tile's interior rows (int32 [ntiles][tile]), out[6]
+ * each entry's in-tile row code (uint16 [nnz]), out[7] the slots each unit
+ * uses (int32 [units]), out[8] each tile's interior row count (int32
+ * [ntiles]), out[9] each tile's flags (uint8 [ntiles]: 1 head row, 2 tail
+ * row, 4 walks). */
+int32_t segtaco_cs_workspace_layout(const segtaco_csr_t *A, int32_t N,
+                                    const segtaco_plan_t *plan, int64_t *out);
+
 /* rows[p] for every stored entry p < nnz: its row as the eb_pr schedule
  * derives it inside tiles of `tile` entries (per 128-entry chunk, a
  * bisection in a staged window of the row pointer, continued in memory for

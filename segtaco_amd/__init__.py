@@ -199,6 +199,10 @@ def lib() -> ctypes.CDLL:
     L.segtaco_spmm_workspace_layout.argtypes = [ctypes.POINTER(CSR), ctypes.c_int32,
                                                 ctypes.POINTER(Plan),
                                                 ctypes.POINTER(ctypes.c_int64)]
+    L.segtaco_cs_workspace_layout.restype = ctypes.c_int32
+    L.segtaco_cs_workspace_layout.argtypes = [ctypes.POINTER(CSR), ctypes.c_int32,
+                                              ctypes.POINTER(Plan),
+                                              ctypes.POINTER(ctypes.c_int64)]
     L.segtaco_entry_rows.restype = ctypes.c_int32
     L.segtaco_entry_rows.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
                                      _vp, _vp]
@@ -688,6 +692,15 @@ def workspace_layout(A: CSR, N: int, plan: Plan) -> dict:
     check(lib().segtaco_spmm_workspace_layout(ctypes.byref(A), N, ctypes.byref(plan), out))
     return dict(ntiles=int(out[0]), fix_off=int(out[1]), rows_off=int(out[2]),
                 head_off=int(out[3]), tail_off=int(out[4]))
+
+
+def cs_workspace_layout(A: CSR, N: int, plan: Plan) -> dict:
+    """Where an eb_cs call keeps its preparation in the workspace."""
+    out = (ctypes.c_int64 * 10)()
+    check(lib().segtaco_cs_workspace_layout(ctypes.byref(A), N, ctypes.byref(plan), out))
+    keys = ("units", "cap", "packed", "rec_off", "slot_rows_off", "int_rows_off",
+            "codes_off", "n_slots_off", "tile_count_off", "tile_flags_off")
+    return {k: int(v) for k, v in zip(keys, out)}
 
 
 def entry_rows(pos_ptr: int, M: int, nnz: int, T: int, tile_rows_ptr: int, rows_ptr: int,

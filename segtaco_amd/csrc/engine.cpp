@@ -365,6 +365,24 @@ int32_t spmm_workspace_layout_impl(const segtaco_csr_t *A, int32_t N,
   return SEGTACO_OK;
 }
 
+int32_t cs_workspace_layout_impl(const segtaco_csr_t *A, int32_t N,
+                                 const segtaco_plan_t *plan, int64_t *out) {
+  if (!A || !plan || !out)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_cs_workspace_layout: null argument");
+  if (plan->schedule != SEGTACO_EB_CS)
+    return set_error(SEGTACO_ERR_UNSUPPORTED,
+                     "segtaco_cs_workspace_layout: schedule %d is not eb_cs", plan->schedule);
+  if (plan->tile < 8 || plan->vec < 1 || plan->group < 1)
+    return set_error(SEGTACO_ERR_INVALID, "segtaco_cs_workspace_layout: unresolved plan");
+  const CsLayout L = cs_layout(A->nnz, A->K, N, plan->tile, plan->group, plan->vec);
+  const int64_t v[10] = {L.units, L.cap, L.packed,
+                         (int64_t)L.rec_off, (int64_t)L.rows_off, (int64_t)L.int_off,
+                         (int64_t)L.code_off, (int64_t)L.nslot_off, (int64_t)L.count_off,
+                         (int64_t)L.flag_off};
+  for (int i = 0; i < 10; ++i) out[i] = v[i];
+  return SEGTACO_OK;
+}
+
 namespace {
 
 // What an eb_cs workspace holds: the product whose preparation (slot plan
@@ -675,6 +693,11 @@ size_t segtaco_spmm_workspace_bytes(const segtaco_csr_t *A, int32_t N,
 int32_t segtaco_spmm_workspace_layout(const segtaco_csr_t *A, int32_t N,
                                       const segtaco_plan_t *plan, int64_t *out) {
   return spmm_workspace_layout_impl(A, N, plan, out);
+}
+
+int32_t segtaco_cs_workspace_layout(const segtaco_csr_t *A, int32_t N,
+                                    const segtaco_plan_t *plan, int64_t *out) {
+  return cs_workspace_layout_impl(A, N, plan, out);
 }
 
 int32_t segtaco_tile_rows(const int32_t *pos, int32_t M, int32_t tile, int32_t ntiles,

@@ -47,11 +47,18 @@ namespace segtaco {
 
 namespace {
 
-constexpr int32_t kCsPosSlack = 8;
 constexpr int kCsPrepWaves = kCsPrepThreads / 64;
 // Bits of an entry's place in its tile inside a sort key (kCsMaxTile = 2^9).
 constexpr int kCsIdxBits = 9;
 static_assert((1 << kCsIdxBits) == kCsMaxTile, "sort key index field");
+// The sort's column buckets (one per lane) and the most keys a lane sorts in
+// registers; a tile with a fuller bucket (its columns clustered on a few
+// values) is sorted by the whole wave instead.
+constexpr int kCsBuckets = 64;
+constexpr int kCsBucketMax = 16;
+// Mark of the tail row's first entry: above every interior mark (k + 1 for
+// interior row k < kCsMaxTile).
+constexpr int32_t kCsTailMark = 1 << 20;
 
 // Record of one entry when columns do not fit the packed form (one 12-byte
 // load).
@@ -59,101 +66,29 @@ struct CsWide {
   uint32_t col, code, vbits;
 };
 
-// Row of entry p given the staged window s_pos[0..pw) = pos[rbase..] (falls
-// back to a bisection of pos in [rbase + pw - 1, r_hi] past the window).
-__device__ __forceinline__ int32_t cs_row(const int32_t *s_pos, int32_t pw,
-                                          int32_t rbase, const int32_t *pos,
-                                          int32_t r_hi, int32_t p) {
-  int32_t lo = 0, hi = pw - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (s_pos[mid] <= p)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return (lo == pw - 1) ? row_of(pos, rbase + lo, r_hi, p) : rbase + lo;
+// x of the lane a DPP row shift (CTRL) names; 0 where that lane lies outside
+// the row of 16 lanes.
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_or_zero(int32_t x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
 }
 
-__device__ __forceinline__ int32_t cs_P(const int32_t *s_pos, int32_t pw,
-                                        int32_t rbase, const int32_t *pos,
-                                        int32_t row) {
-  const int32_t i = row - rbase;
-  return i < pw ? s_pos[i] : pos[row];
-}
-
-// Wave-wide inclusive prefix sum of x (64 lanes).
-__device__ __forceinline__ int32_t wave_incl_scan(int32_t x, int lane) {
-  for (int d = 1; d < 64; d <<= 1) {
-    const int32_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
-}
-
-// Wave-wide sum of x.
-__device__ __forceinline__ int32_t wave_sum(int32_t x) {
-  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-  return x;
-}
-
-// One tile as the prep kernel sees it: each lane holds entries lane + 64k
-// (k < E), their rows, and whether the tile has a head / tail row.
-template <int E>
-struct PrepTile {
-  int32_t s, e, cn;
-  int32_t pw;  // rows staged in the wave's window from r_first
-  int32_t row[E];
-  bool head, tail;
-  int32_t r_first, r_last;
-};
-
-// Rows of tile t's entries and its head / tail rows. s_pos: this wave's
-// LDS window of the row pointer (kCsMaxTile + kCsPosSlack entries).
-template <int E>
-__device__ void cs_tile_rows(const CsArgs &a, int32_t t, int32_t *s_pos, PrepTile<E> &P) {
-  const int lane = threadIdx.x & 63;
-  const int32_t *__restrict__ pos = a.pos;
-  P.s = t * a.tile;
-  P.e = (int32_t)min((int64_t)a.nnz, (int64_t)P.s + a.tile);
-  P.cn = P.e - P.s;
-  const int32_t rbase = a.tile_row[t];
-  const int32_t r_hi = min(a.M - 1, a.tile_row[t + 1]);
-  const int32_t pw = min(a.tile + kCsPosSlack, a.M + 1 - rbase);
-  P.pw = pw;
-  for (int32_t i = lane; i < pw; i += 64) s_pos[i] = pos[rbase + i];
-  wave_sync();
-#pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const int32_t i = lane + 64 * k;
-    P.row[k] = i < P.cn ? cs_row(s_pos, pw, rbase, pos, r_hi, P.s + i) : INT_MAX;
-  }
-  P.r_first = rbase;  // the repaired row of the tile's first entry
-  // the row of the last entry: the lane / register holding entry cn - 1
-  const int32_t last = P.cn - 1;
-  int32_t rl = 0;
-#pragma unroll
-  for (int k = 0; k < E; ++k)
-    if (last / 64 == k) rl = __shfl(P.row[k], last & 63, 64);
-  P.r_last = rl;
-  P.head = cs_P(s_pos, pw, rbase, pos, P.r_first) < P.s;
-  P.tail = !(P.head && P.r_last == P.r_first) && cs_P(s_pos, pw, rbase, pos, P.r_last + 1) > P.e;
-  // callers read s_pos again: it stays valid until the next tile
-}
-
-// Interior rows of the tile: those that start and end inside it. Entry i
-// starts one when its row begins at i and is neither the head nor the tail.
-template <int E>
-__device__ __forceinline__ bool cs_starts_interior(const CsArgs &a, const PrepTile<E> &P,
-                                                   const int32_t *s_pos, int k) {
-  const int lane = threadIdx.x & 63;
-  const int32_t i = lane + 64 * k;
-  if (i >= P.cn) return false;
-  const int32_t r = P.row[k];
-  if (cs_P(s_pos, P.pw, P.r_first, a.pos, r) != P.s + i) return false;
-  if (P.head && r == P.r_first) return false;
-  if (P.tail && r == P.r_last) return false;
-  return true;
+// Inclusive scan over the wave's 64 lanes, sum (kMax false) or maximum
+// (kMax true), of values >= 0 (0 is the identity of both): shifts by 1, 2,
+// 4 and 8 lanes inside each row of 16 (DPP, no LDS), then each row adds the
+// totals of the rows below it (lanes 15, 31 and 47).
+template <bool kMax>
+__device__ __forceinline__ int32_t wave_scan(int32_t x) {
+  auto op = [](int32_t u, int32_t v) { return kMax ? max(u, v) : u + v; };
+  x = op(x, dpp_or_zero<0x111>(x));  // row_shr:1
+  x = op(x, dpp_or_zero<0x112>(x));  // row_shr:2
+  x = op(x, dpp_or_zero<0x114>(x));  // row_shr:4
+  x = op(x, dpp_or_zero<0x118>(x));  // row_shr:8
+  const int32_t b1 = __builtin_amdgcn_readlane(x, 15);
+  const int32_t b2 = op(b1, __builtin_amdgcn_readlane(x, 31));
+  const int32_t b3 = op(b2, __builtin_amdgcn_readlane(x, 47));
+  const int row = (threadIdx.x & 63) >> 4;
+  return row == 0 ? x : op(x, row == 1 ? b1 : row == 2 ? b2 : b3);
 }
 
 // Slot plan of tile t's unit. The main kernel's wave w runs tiles 8w ..
@@ -167,22 +102,23 @@ __device__ int32_t cs_slot_base(const CsArgs &a, int32_t t, int32_t &n_slots, in
   const int32_t t0 = (t / a.group) * a.group;
   const int32_t g = t - t0;
   const int32_t cnt = (lane < a.group && t0 + lane < a.ntiles) ? a.tile_count[t0 + lane] : 0;
-  // interior rows of the tiles before g in g's wave, and of each wave
+  // interior rows of the unit's tiles 0 .. j, and of none for j = -1
+  const int32_t inc = wave_scan<false>(cnt);
+  auto upto = [&](int32_t j) { return j < 0 ? 0 : __builtin_amdgcn_readlane(inc, j); };
   const int32_t w_g = g >> 3;
-  const int32_t before = wave_sum((lane >> 3) == w_g && lane < g ? cnt : 0);
-  int32_t seg = cnt;
-  for (int d = 4; d > 0; d >>= 1) seg += __shfl_xor(seg, d, 64);
+  const int32_t before = upto(g - 1) - upto(8 * w_g - 1);  // tiles before g in its wave
   const int32_t room = a.cap - 2 * a.group;
   int32_t used = 0, base = -1;
+#pragma unroll
   for (int w = 0; w < 8; ++w) {
-    const int32_t sw = __shfl(seg, 8 * w, 64);
+    const int32_t sw = upto(8 * w + 7) - upto(8 * w - 1);
     if (used + sw <= room) {
       if (w == w_g) base = 2 * a.group + used + before;
       used += sw;
     }
   }
   n_slots = 2 * a.group + used;
-  count = __shfl(cnt, g, 64);
+  count = __builtin_amdgcn_readlane(cnt, g);
   return base;
 }
 
@@ -222,64 +158,127 @@ __device__ __forceinline__ void wave_bitonic(uint32_t (&key)[R]) {
   }
 }
 
-// Prep, one wave per tile (many waves per CU hide the latency of the
-// per-entry row searches). k_cs_count: per tile its head / tail flags, its
-// interior rows (count, and the row of the k-th in int_rows) and per entry
-// the tile-local code of its row (0 head, 1 tail, 2 + k the k-th interior
-// row). k_cs_sort: the unit's slot plan (cs_slot_base), then the records:
-// walking tiles in CSR order with the tile-local codes, sweeping tiles
-// sorted by (column, position) with the codes mapped to LDS slots, and the
-// rows of the interior slots. Every load k_cs_sort makes is issued before
-// anything waits, so the kernel costs one memory round trip.
+// Ascending sort of W <= 16 keys in one lane's registers: Batcher's odd-even
+// merge network for the next power of two without the compare-exchanges
+// that reach past W (they would meet +infinity padding, which stays on top):
+// 19, 32, 42 and 63 compare-exchanges for W = 8, 10, 12 and 16.
+template <int W>
+__device__ __forceinline__ void lane_sort(uint32_t (&v)[W]) {
+#pragma unroll
+  for (int p = 1; p < W; p <<= 1) {
+#pragma unroll
+    for (int k = p; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % p; j + k < W; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i + j + k < W && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+            const uint32_t x = v[i + j], y = v[i + j + k];
+            v[i + j] = min(x, y);
+            v[i + j + k] = max(x, y);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Lane b sorts bucket b in place: sk[st .. st + n), n <= W.
+template <int W>
+__device__ __forceinline__ void cs_sort_bucket(uint32_t *sk, int32_t st, int32_t n) {
+  uint32_t v[W];
+#pragma unroll
+  for (int m = 0; m < W; ++m) v[m] = m < n ? sk[st + m] : 0xffffffffu;
+  lane_sort<W>(v);
+#pragma unroll
+  for (int m = 0; m < W; ++m)
+    if (m < n) sk[st + m] = v[m];
+}
+
+// Prep, one wave per tile.
+//
+// k_cs_count: the tile's rows, 64 per step from its first row (the repaired
+// tile row) until the rows begin past the tile. Per row: head (began before
+// the tile), tail (begins inside, ends past it) or interior (begins and ends
+// inside, not empty); interior rows are numbered in row order and their ids
+// go to int_rows. Each row that begins inside marks its first entry in LDS
+// (k + 1 for interior row k, kCsTailMark for the tail), and a running
+// maximum over the tile's entries names every entry's row: code 0 head, 1
+// tail, 2 + k interior row k. Then the tile's count of interior rows and its
+// head / tail flags.
 template <int E>
 __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
-  __shared__ int32_t s_pos[kCsPrepWaves][kCsMaxTile + kCsPosSlack];
+  __shared__ int32_t s_mark[kCsPrepWaves][kCsMaxTile];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int32_t t = blockIdx.x * kCsPrepWaves + wave;
   if (t >= a.ntiles) return;  // uniform over the wave
-  int32_t *sp = s_pos[wave];
-  PrepTile<E> P;
-  cs_tile_rows<E>(a, t, sp, P);
-  int32_t code[E];
-  int32_t running = 0;  // interior rows started by earlier chunks
+  int32_t *mk = s_mark[wave];
+  const int32_t s = t * a.tile;
+  const int32_t cn = min(a.nnz - s, a.tile);
+  const int32_t e = s + cn;
 #pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const bool st = cs_starts_interior<E>(a, P, sp, k);
-    const int32_t inc = wave_incl_scan(st ? 1 : 0, lane) + running;
-    running = __shfl(inc, 63, 64);
-    // the entry that starts an interior row names the row's place k in the
-    // tile; the row's other entries take it from the running maximum below
-    code[k] = st ? inc - 1 : -1;
-    if (st) a.int_rows[(int64_t)t * a.tile + inc - 1] = P.row[k];
-  }
-  int32_t carry = -1;
-#pragma unroll
-  for (int k = 0; k < E; ++k) {
-    int32_t m = code[k];
-    for (int d = 1; d < 64; d <<= 1) {
-      const int32_t y = __shfl_up(m, d, 64);
-      if (lane >= d) m = max(m, y);
+  for (int k = 0; k < E; ++k)
+    if (lane + 64 * k < cn) mk[lane + 64 * k] = 0;
+  wave_sync();
+  const int32_t r_hi = min(a.M - 1, a.tile_row[t + 1]);
+  int32_t count = 0;
+  bool head = false, tail = false;
+  for (int32_t rb = a.tile_row[t]; rb <= r_hi; rb += 64) {
+    const int32_t r = rb + lane;
+    int32_t p0 = e, p1 = e;  // past r_hi: as an empty row at the tile's end
+    if (r <= r_hi) {
+      p0 = a.pos[r];
+      p1 = a.pos[r + 1];
     }
-    m = max(m, carry);
-    carry = __shfl(m, 63, 64);
+    const bool inside = p0 >= s && p0 < e;
+    const bool intr = inside && p1 <= e && p1 > p0;
+    const bool tl = inside && p1 > e;
+    head = head || __ballot(p0 < s && p1 > s) != 0;
+    tail = tail || __ballot(tl) != 0;
+    const uint64_t im = __ballot(intr);
+    const int32_t k = count + (int32_t)__builtin_amdgcn_mbcnt_hi(
+                                  (uint32_t)(im >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)im, 0u));
+    if (intr) {
+      a.int_rows[(int64_t)t * a.tile + k] = r;
+      mk[p0 - s] = k + 1;
+    }
+    if (tl) mk[p0 - s] = kCsTailMark;
+    count += __popcll(im);
+    if (__builtin_amdgcn_readlane(p1, 63) >= e) break;  // later rows begin past the tile
+  }
+  wave_sync();
+  int32_t carry = 0;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
     const int32_t i = lane + 64 * k;
-    const int32_t r = P.row[k];
-    const bool hd = P.head && r == P.r_first;
-    const bool tl = !hd && P.tail && r == P.r_last;
-    if (i < P.cn) a.codes[P.s + i] = (uint16_t)(hd ? 0 : tl ? 1 : 2 + m);
+    const int32_t m = max(wave_scan<true>(i < cn ? mk[i] : 0), carry);
+    carry = __builtin_amdgcn_readlane(m, 63);
+    if (i < cn)
+      a.codes[s + i] = (uint16_t)(m == 0 ? 0 : m == kCsTailMark ? 1 : min(m, count) + 1);
   }
   if (lane == 0) {
-    a.tile_count[t] = running;
-    a.tile_flags[t] = (uint8_t)((P.head ? 1 : 0) | (P.tail ? 2 : 0));
+    a.tile_count[t] = count;
+    a.tile_flags[t] = (uint8_t)((head ? 1 : 0) | (tail ? 2 : 0));
   }
 }
 
+// k_cs_sort: the unit's slot plan (cs_slot_base), then the records. A
+// walking tile keeps CSR order and the tile-local codes. A sweeping tile maps
+// its codes to LDS slots, writes the rows of its interior slots, and sorts
+// its entries by (column, position): each entry goes to one of 64 buckets of
+// the tile's column range (a monotone map of the column), a count and a
+// prefix sum place the buckets, and lane b sorts bucket b in registers (a
+// bucket above kCsBucketMax keys sends the tile to a whole-wave bitonic
+// sort). The keys (column << 9 | place) are unique, so the order does not
+// depend on the order the buckets were filled in. Every load is issued
+// before anything waits.
 template <int E>
 __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
-  constexpr int R = E <= 1 ? 1 : E <= 2 ? 2 : E <= 4 ? 4 : 8;  // sort registers
-  __shared__ int32_t s_code[kCsPrepWaves][kCsMaxTile];
-  __shared__ uint32_t s_val[kCsPrepWaves][kCsMaxTile];
+  constexpr int R = E <= 1 ? 1 : E <= 2 ? 2 : E <= 4 ? 4 : 8;  // whole-wave sort registers
+  __shared__ uint32_t s_key[kCsPrepWaves][kCsMaxTile];
+  __shared__ uint2 s_rec[kCsPrepWaves][kCsMaxTile];
+  __shared__ int32_t s_bkt[kCsPrepWaves][kCsBuckets];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int32_t t = blockIdx.x * kCsPrepWaves + wave;
@@ -287,20 +286,31 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
   const int32_t unit = t / a.group;
   const int32_t g = t - unit * a.group;
   const int32_t s = t * a.tile;
-  const int32_t cn = (int32_t)min((int64_t)a.nnz - s, (int64_t)a.tile);
-  // every load of the tile at once: codes, columns, values, interior rows
+  const int32_t cn = min(a.nnz - s, a.tile);
+  const uint16_t *codes = a.codes + s;
+  const int32_t *crd = a.crd + s;
+  const int32_t *irows = a.int_rows + (int64_t)t * a.tile;
   int32_t cd[E], col[E], irow[E];
   uint32_t vb[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int32_t i = min(lane + 64 * k, cn - 1);
-    cd[k] = a.codes[s + i];
-    col[k] = a.crd[s + i];
-    vb[k] = a.one_bits;  // A without values reads 1.0
-    if (a.vals)
-      vb[k] = a.elem == 4 ? static_cast<const uint32_t *>(a.vals)[s + i]
-                          : (uint32_t) static_cast<const uint16_t *>(a.vals)[s + i];
-    irow[k] = a.int_rows[(int64_t)t * a.tile + i];
+    cd[k] = codes[i];
+    col[k] = crd[i];
+    irow[k] = irows[i];
+  }
+  // value bits: the storage type is uniform, so one branch around all loads
+  if (a.vals == nullptr) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) vb[k] = a.one_bits;  // A without values reads 1.0
+  } else if (a.elem == 4) {
+    const uint32_t *v = static_cast<const uint32_t *>(a.vals) + s;
+#pragma unroll
+    for (int k = 0; k < E; ++k) vb[k] = v[min(lane + 64 * k, cn - 1)];
+  } else {
+    const uint16_t *v = static_cast<const uint16_t *>(a.vals) + s;
+#pragma unroll
+    for (int k = 0; k < E; ++k) vb[k] = v[min(lane + 64 * k, cn - 1)];
   }
   int32_t n_slots = 0, count = 0;
   const int32_t base = cs_slot_base(a, t, n_slots, count);
@@ -333,30 +343,76 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
   }
   if (a.packed) {
     // columns below 2^(32 - kCsCodeBits) <= 2^20: the key (column << 9 |
-    // place) is unique, fits 29 bits and orders by (column, position);
-    // places past the tile sort last
-    int32_t *sc = s_code[wave];
-    uint32_t *sv = s_val[wave];
-    uint32_t key[R];
+    // place) is unique and fits 29 bits; the records wait in LDS by place
+    uint32_t *sk = s_key[wave];
+    uint2 *sr = s_rec[wave];
+    int32_t *nb = s_bkt[wave];
+    int32_t hi = 0, flip = 0;  // largest column; largest INT_MAX - column
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int32_t i = lane + 64 * r;
-      key[r] = 0xffffffffu;
-      if (r < E && i < cn) {
-        key[r] = ((uint32_t)col[r] << kCsIdxBits) | (uint32_t)i;
-        sc[i] = cd[r];
-        sv[i] = vb[r];
+    for (int k = 0; k < E; ++k) {
+      const int32_t i = lane + 64 * k;
+      if (i < cn) {
+        hi = max(hi, col[k]);
+        flip = max(flip, INT_MAX - col[k]);
+        sr[i] = make_uint2(((uint32_t)col[k] << kCsCodeBits) | (uint32_t)cd[k], vb[k]);
+      }
+    }
+    hi = __builtin_amdgcn_readlane(wave_scan<true>(hi), 63);
+    const int32_t lo = INT_MAX - __builtin_amdgcn_readlane(wave_scan<true>(flip), 63);
+    // bucket of a column: (column - lo) * 64 / span, rounded down in FP32 —
+    // exact inputs (span <= 2^20) and a positive factor keep it monotone
+    const float scale = (float)kCsBuckets / (float)(hi - lo + 1);
+    int32_t bk[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+      bk[k] = min(kCsBuckets - 1, (int32_t)((float)(col[k] - lo) * scale));
+    nb[lane] = 0;
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < E; ++k)
+      if (lane + 64 * k < cn) atomicAdd(&nb[bk[k]], 1);
+    wave_sync();
+    const int32_t n = nb[lane];                   // keys in bucket `lane`
+    const int32_t st = wave_scan<false>(n) - n;   // its first place
+    const int32_t n_max = __builtin_amdgcn_readlane(wave_scan<true>(n), 63);
+    nb[lane] = st;
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int32_t i = lane + 64 * k;
+      if (i < cn) sk[atomicAdd(&nb[bk[k]], 1)] = ((uint32_t)col[k] << kCsIdxBits) | (uint32_t)i;
+    }
+    wave_sync();
+    // the network for the fullest bucket (19, 32, 42 or 63 compare-exchanges)
+    if (n_max <= 8) {
+      cs_sort_bucket<8>(sk, st, n);
+    } else if (n_max <= 10) {
+      cs_sort_bucket<10>(sk, st, n);
+    } else if (n_max <= 12) {
+      cs_sort_bucket<12>(sk, st, n);
+    } else if (n_max <= kCsBucketMax) {
+      cs_sort_bucket<kCsBucketMax>(sk, st, n);
+    } else {
+      uint32_t key[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int32_t i = lane + 64 * r;
+        key[r] = 0xffffffffu;
+        if (r < E && i < cn) key[r] = ((uint32_t)col[r] << kCsIdxBits) | (uint32_t)i;
+      }
+      wave_bitonic<R>(key);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int32_t d = lane + 64 * r;
+        if (d < cn) sk[d] = key[r];
       }
     }
     wave_sync();
-    wave_bitonic<R>(key);
+    uint2 *out = static_cast<uint2 *>(a.s_rec) + s;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int32_t d = lane + 64 * r;
-      if (d < cn) {
-        const int32_t i = (int32_t)(key[r] & (kCsMaxTile - 1));
-        put(d, (int32_t)(key[r] >> kCsIdxBits), sc[i], sv[i]);
-      }
+    for (int k = 0; k < E; ++k) {
+      const int32_t d = lane + 64 * k;
+      if (d < cn) out[d] = sr[sk[d] & (kCsMaxTile - 1)];
     }
     return;
   }

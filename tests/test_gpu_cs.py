@@ -200,6 +200,184 @@ def test_prepared_calls_match_full_calls(sched):
     assert np.array_equal(C.cpu().numpy().view(np.uint32), C_full.view(np.uint32))
 
 
+def _vbits(vals, dtype, nnz):
+    """Value bits a record carries per entry: the FP32 word, or the 16-bit
+    storage word zero-extended; 1.0 in the storage type without values."""
+    if vals is None:
+        one = {"f32": 0x3F800000, "bf16": 0x3F80, "fp16": 0x3C00}[dtype]
+        return np.full(nnz, one, np.uint32)
+    if dtype == "f32":
+        return np.asarray(vals, np.float32).view(np.uint32)
+    return S.to_half(dtype, vals).view(np.uint16).astype(np.uint32)
+
+
+def _check_preparation(pos, crd, vals, K, N, dtype="f32", **kw):
+    """segtaco_spmm_prepare for eb_cs, then every array it leaves in the
+    workspace checked against what the CSR itself fixes: each tile's head /
+    tail rows and interior rows (non-empty rows whose entries all lie in the
+    tile), each entry's in-tile row code, the per-wave sweep / walk choice,
+    and the records — a walking tile's in CSR order, a sweeping tile's a
+    permutation of its entries, ordered by (column, row), whose slots name
+    the entries' own rows, with no slot shared by two tiles. Returns the
+    resolved plan and the share of tiles that walk."""
+    t = torch()
+    pos = np.asarray(pos, np.int64)
+    crd = np.asarray(crd, np.int64)
+    A = DeviceCSR(pos, crd, vals, K, dtype=dtype)
+    bd = S.DTYPES[dtype]
+    p = S.plan_resolve(A.desc, N, bd, S.make_plan("eb_cs", **kw), N, N)
+    need = S.workspace_bytes(A.desc, N, p)
+    ws = t.zeros(need, dtype=t.uint8, device="cuda")
+    S.spmm_prepare(A.desc, bd, N, N, N, p, ws.data_ptr(), need, stream_ptr())
+    t.cuda.synchronize()
+    eb = S.workspace_layout(A.desc, N, p)
+    cs = S.cs_workspace_layout(A.desc, N, p)
+    w = ws.cpu().numpy()
+
+    def arr(off, dt, n):
+        return np.frombuffer(w, dtype=dt, count=n, offset=off).astype(np.int64)
+
+    M, nnz, T, G, cap = len(pos) - 1, int(pos[-1]), p.tile, p.group, cs["cap"]
+    nt = eb["ntiles"]
+    assert nt == -(-nnz // T) and cs["units"] == -(-nt // G)
+    tile_row = arr(eb["rows_off"], np.int32, nt + 1)
+    flags = arr(cs["tile_flags_off"], np.uint8, nt)
+    count = arr(cs["tile_count_off"], np.int32, nt)
+    n_slots = arr(cs["n_slots_off"], np.int32, cs["units"])
+    slot_rows = arr(cs["slot_rows_off"], np.int32, cs["units"] * cap)
+    int_rows = arr(cs["int_rows_off"], np.int32, nt * T)
+    codes = arr(cs["codes_off"], np.uint16, nnz)
+
+    row = np.repeat(np.arange(M), np.diff(pos))  # row of every entry
+    tile = np.arange(nnz) // T
+    s = np.arange(nt) * T
+    e = np.minimum(s + T, nnz)
+    first, last = row[s], row[e - 1]
+    assert np.array_equal(tile_row[:nt], first)
+    head = pos[first] < s
+    tail = (pos[last + 1] > e) & ~(head & (last == first))
+    assert np.array_equal(flags & 3, head.astype(np.int64) | (tail.astype(np.int64) << 1))
+    ne = np.flatnonzero(np.diff(pos) > 0)
+    interior = ne[pos[ne] // T == (pos[ne + 1] - 1) // T]
+    it = pos[interior] // T
+    assert np.array_equal(count, np.bincount(it, minlength=nt))
+    k_of = np.arange(len(interior)) - np.searchsorted(it, it)
+    assert np.array_equal(int_rows[it * T + k_of], interior)
+    kk = np.full(M, -1)
+    kk[interior] = k_of
+    in_head = head[tile] & (row == first[tile])
+    in_tail = ~in_head & tail[tile] & (row == last[tile])
+    exp_code = np.where(in_head, 0, np.where(in_tail, 1, 2 + kk[row]))
+    assert (exp_code >= 0).all()
+    assert np.array_equal(codes, exp_code)
+
+    # the slot plan: per unit, waves of 8 tiles in order sweep while their
+    # interior rows fit beside the earlier sweeping waves'
+    walk = (flags & 4) != 0
+    for u in range(cs["units"]):
+        used = 0
+        for wv in range(8):
+            lo, hi = u * G + 8 * wv, min(u * G + min(8 * wv + 8, G), nt)
+            if lo >= hi:
+                continue
+            seg = int(count[lo:hi].sum())
+            fits = used + seg <= cap - 2 * G
+            assert (~walk[lo:hi]).all() if fits else walk[lo:hi].all(), (u, wv)
+            used += seg if fits else 0
+        assert n_slots[u] == 2 * G + used <= cap
+
+    vb = _vbits(vals, dtype, nnz)
+    words = 2 if cs["packed"] else 3
+    raw = arr(cs["rec_off"], np.uint32, nnz * words).reshape(nnz, words)
+    if cs["packed"]:
+        rcol, rcode, rval = raw[:, 0] >> 12, raw[:, 0] & 4095, raw[:, 1]
+    else:
+        rcol, rcode, rval = raw[:, 0], raw[:, 1], raw[:, 2]
+    wt = walk[tile]
+    assert np.array_equal(rcol[wt], crd[wt])
+    assert np.array_equal(rval[wt], vb[wt])
+    assert np.array_equal(rcode[wt], codes[wt])
+    sw = ~wt
+    st, sc, sv, sk = tile[sw], rcol[sw], rval[sw], rcode[sw]
+    unit, g = st // G, st % G
+    assert (sk < n_slots[unit]).all()
+    assert ((sk >= 2 * G) | (sk == 2 * g) | (sk == 2 * g + 1)).all()
+    assert (head[st] | (sk != 2 * g)).all() and (tail[st] | (sk != 2 * g + 1)).all()
+    rrow = np.where(sk == 2 * g, first[st],
+                    np.where(sk == 2 * g + 1, last[st], slot_rows[unit * cap + sk]))
+    # a permutation of each tile's entries ...
+    o1 = np.lexsort((sc, rrow, st))
+    o2 = np.lexsort((crd[sw], row[sw], tile[sw]))
+    assert np.array_equal(st[o1], tile[sw][o2])
+    assert np.array_equal(rrow[o1], row[sw][o2])
+    assert np.array_equal(sc[o1], crd[sw][o2])
+    assert np.array_equal(sv[o1], vb[sw][o2])
+    # ... in (column, row) order ...
+    same = st[1:] == st[:-1]
+    up = (sc[1:] > sc[:-1]) | ((sc[1:] == sc[:-1]) & (rrow[1:] > rrow[:-1]))
+    assert (up | ~same).all()
+    # ... and no interior slot serves two tiles
+    isl = sk >= 2 * G
+    key = unit[isl] * cap + sk[isl]
+    o = np.argsort(key, kind="stable")
+    dup = key[o][1:] == key[o][:-1]
+    assert (st[isl][o][1:][dup] == st[isl][o][:-1][dup]).all()
+    return p, float(walk.mean())
+
+
+@pytest.mark.parametrize("case", ["powerlaw", "banded", "one_column_in_every_row",
+                                  "one_entry_rows", "mixed", "wide", "no_values", "bf16"])
+def test_preparation_arrays(case):
+    """The preparation's arrays on structures that take each of its paths:
+    the 8- and 16-key bucket sorts (power-law, banded), the whole-wave sort
+    (a column shared by every row fills one bucket), walking units, units
+    mixing both, 96-bit records, A without values, 16-bit values."""
+    rng = np.random.default_rng(31)
+    kw, K, dtype, vals_none = {}, 30000, "f32", False
+    if case in ("powerlaw", "no_values", "bf16"):
+        pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 5, 30000, 30000)
+        vals_none = case == "no_values"
+        dtype = "bf16" if case == "bf16" else "f32"
+        kw = dict(tile=256) if case == "powerlaw" else {}
+    elif case == "banded":
+        pos, crd, vals = S.gen_csr(S.GEN_BANDED, 3, 20000, 20000, bw=8)
+        K = 20000
+    elif case == "one_column_in_every_row":
+        lens = rng.integers(1, 6, 20000)
+        pos, crd, vals, _ = _mk(rng, lens, 30000, 1)
+        crd[pos[:-1]] = 0  # each row's first entry in column 0 (rows stay sorted)
+        kw = dict(tile=256)  # ~85 rows, so ~85 keys in one bucket, per tile
+    elif case == "one_entry_rows":
+        M = 70000
+        pos = np.arange(M + 1, dtype=np.int32)
+        crd = rng.integers(0, 5000, M).astype(np.int32)
+        vals = rng.uniform(-1, 1, M).astype(np.float32)
+        K, kw = 5000, dict(tile=512)
+    elif case == "mixed":
+        lens = np.concatenate([np.ones(60000, np.int64), rng.integers(20, 60, 6000)])
+        pos, crd, vals, _ = _mk(rng, lens, 30000, 1)
+        kw = dict(tile=128)
+    else:  # wide
+        K = (1 << 20) + 4099
+        lens = np.concatenate([rng.integers(0, 30, 8000), np.ones(20000, np.int64)])
+        pos, crd, vals, _ = _mk(rng, lens, K, 1)
+        kw = dict(tile=256)
+    p, walked = _check_preparation(pos, crd, None if vals_none else vals, K, 64, dtype, **kw)
+    print(f"[cs prep] {case}: plan {p}, walking tiles {walked:.3f}", flush=True)
+    if case == "one_entry_rows":
+        # 8 tiles of 512 one-entry rows overflow the slots: every full wave
+        # walks (only the last unit's lone 368-entry tile fits and sweeps)
+        assert walked > 0.99
+    if case == "mixed":
+        assert 0.0 < walked < 1.0
+
+
+def test_preparation_arrays_full_size_powerlaw():
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 2, 262144, 262144)
+    p, walked = _check_preparation(pos, crd, vals, 262144, 128)
+    print(f"[cs prep full] plan {p}, walking tiles {walked:.3f}", flush=True)
+
+
 def test_batch_refuses_eb_cs():
     t = torch()
     pos, crd, vals = S.gen_csr(S.GEN_UNIFORM, 1, 500, 500, mean=8)
