@@ -208,9 +208,9 @@ __device__ __forceinline__ void cs_sort_bucket(uint32_t *sk, int32_t st, int32_t
 // head / tail flags.
 template <int E>
 __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
-  __shared__ int32_t s_mark[kCsPrepWaves][kCsMaxTile];
+  __shared__ int32_t s_mark[kCsPrepWaves][64 * E];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar tile math
   const int32_t t = blockIdx.x * kCsPrepWaves + wave;
   if (t >= a.ntiles) return;  // uniform over the wave
   int32_t *mk = s_mark[wave];
@@ -274,13 +274,13 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
 // depend on the order the buckets were filled in. Every load is issued
 // before anything waits.
 template <int E>
-__global__ __launch_bounds__(kCsPrepThreads) void k_cs_sort(const CsArgs a) {
+__global__ __launch_bounds__(kCsPrepThreads, E <= 6 ? 8 : 5) void k_cs_sort(const CsArgs a) {
   constexpr int R = E <= 1 ? 1 : E <= 2 ? 2 : E <= 4 ? 4 : 8;  // whole-wave sort registers
-  __shared__ uint32_t s_key[kCsPrepWaves][kCsMaxTile];
-  __shared__ uint2 s_rec[kCsPrepWaves][kCsMaxTile];
+  __shared__ uint32_t s_key[kCsPrepWaves][64 * E];
+  __shared__ uint2 s_rec[kCsPrepWaves][64 * E];
   __shared__ int32_t s_bkt[kCsPrepWaves][kCsBuckets];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar tile math
   const int32_t t = blockIdx.x * kCsPrepWaves + wave;
   if (t >= a.ntiles) return;  // uniform over the wave
   const int32_t unit = t / a.group;
