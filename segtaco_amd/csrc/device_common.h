@@ -212,6 +212,24 @@ __device__ __forceinline__ void store_vec(float *__restrict__ p,
   }
 }
 
+// store_vec for data nothing reads back soon (non-temporal: not kept in L2),
+// as whole 16- / 8-byte vector stores.
+template <int V>
+__device__ __forceinline__ void store_vec_nt(float *__restrict__ p, const float (&v)[V]) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  if constexpr (V == 8) {
+    __builtin_nontemporal_store(f4{v[0], v[1], v[2], v[3]}, reinterpret_cast<f4 *>(p));
+    __builtin_nontemporal_store(f4{v[4], v[5], v[6], v[7]}, reinterpret_cast<f4 *>(p) + 1);
+  } else if constexpr (V == 4) {
+    __builtin_nontemporal_store(f4{v[0], v[1], v[2], v[3]}, reinterpret_cast<f4 *>(p));
+  } else if constexpr (V == 2) {
+    __builtin_nontemporal_store(f2{v[0], v[1]}, reinterpret_cast<f2 *>(p));
+  } else {
+    __builtin_nontemporal_store(v[0], p);
+  }
+}
+
 template <int V>
 __device__ __forceinline__ void load_f32vec(const float *__restrict__ p,
                                             float (&v)[V]) {

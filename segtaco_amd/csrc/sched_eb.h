@@ -151,15 +151,12 @@ __device__ __forceinline__ void eb_sr_walk(const KArgs &a, int32_t blk,
 #pragma unroll
       for (int j = 0; j < V; ++j) acc[j] += cmp[j];
     }
-    float *dst = before  ? a.carry_head + (int64_t)t * a.N + col
-                 : after ? a.carry_tail + (int64_t)t * a.N + col
-                         : a.C + (int64_t)r * a.ldc + col;
     if (before || after) {
-      store_vec<V>(dst, acc);  // carries: read back by the fix-up
+      // carries: read back by the fix-up
+      store_vec<V>((before ? a.carry_head : a.carry_tail) + (int64_t)t * a.N + col, acc);
     } else {
       // C rows are never read back here: non-temporal, to leave L2 to B
-#pragma unroll
-      for (int j = 0; j < V; ++j) __builtin_nontemporal_store(acc[j], dst + j);
+      store_vec_nt<V>(a.C + (int64_t)r * a.ldc + col, acc);
     }
   };
   auto next_row = [&](int32_t pp) {
@@ -180,7 +177,8 @@ __device__ __forceinline__ void eb_sr_walk(const KArgs &a, int32_t blk,
     const int32_t cn = max(0, min(e - cs, CH));
     if (cn > 0) {
       // the tile's entries are read once: non-temporal (power-law 262144^2,
-      // N = 128, FP32, with the C stores above: 343 vs 352 us per step)
+      // N = 128, FP32, with the C stores above: 343 vs 352 us per step on one
+      // host, the same time within 1 % on others)
       for (int32_t i = q; i < cn; i += Q) {
         s_crd[i] = __builtin_nontemporal_load(crd + cs + i);
         s_val[i] = load_val_stream<DT>(vals, cs + i);
