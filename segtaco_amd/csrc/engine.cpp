@@ -67,8 +67,11 @@ CsLayout cs_layout(int32_t nnz, int32_t K, int32_t N, int32_t tile, int32_t grou
   CsLayout L{};
   L.eb = eb_layout(nnz, N, tile);
   L.units = (int32_t)ceil_div(L.eb.ntiles, group);
-  const size_t budget = device_max_lds() < kCsLdsBudget ? device_max_lds() : kCsLdsBudget;
-  const size_t cap = budget / ((size_t)kCsLanes * vec * sizeof(float));
+  // a slot: kCsLanes * vec FP32 accumulators, and its row id beside them
+  const size_t slot = (size_t)kCsLanes * vec * sizeof(float);
+  size_t cap = kCsLdsBudget / slot;
+  if (cap > device_max_lds() / (slot + sizeof(int32_t)))
+    cap = device_max_lds() / (slot + sizeof(int32_t));
   const size_t max_code = (size_t)1 << kCsCodeBits;  // slots fit the code field
   L.cap = (int32_t)(cap < max_code ? cap : max_code);
   L.packed = K <= (1 << (32 - kCsCodeBits)) ? 1 : 0;

@@ -625,7 +625,7 @@ __device__ __forceinline__ void cs_walk(const CsArgs &a, int32_t t, int32_t s, i
 
 template <int DT, int V, bool kPacked>
 __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
-  extern __shared__ float acc_lds[];  // [cap][W]
+  extern __shared__ float acc_lds[];  // [cap][W] accumulators, then [cap] slot rows
   constexpr int32_t Q = kCsLanes;
   constexpr int32_t W = Q * V;
   const int lane = threadIdx.x & 63;
@@ -644,6 +644,11 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
     const int32_t e = tok ? (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile) : 0;
     const int32_t flags = tok ? a.tile_flags[t] : 0;
     const bool walk = (flags & 4) != 0;  // the same for the wave's tiles
+    // the rows of the interior slots, read once for all panels (the panel
+    // loop's first barrier orders these writes before the first write-out)
+    int32_t *s_rows = reinterpret_cast<int32_t *>(acc_lds + (size_t)a.cap * W);
+    for (int32_t i = threadIdx.x; i < n_slots - 2 * a.group; i += kCsThreads)
+      s_rows[i] = a.slot_rows[(int64_t)unit * a.cap + 2 * a.group + i];
     for (int32_t panel = 0; panel < panels; ++panel) {
       const int32_t col = panel * W + q * V;
       const bool colok = col < a.N;
@@ -677,7 +682,7 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
         const int32_t seg = i % PER;
         const int32_t c = panel * W + seg * V;
         if (c < a.N) {
-          const int32_t row = a.slot_rows[(int64_t)unit * a.cap + sl];
+          const int32_t row = s_rows[sl - first];
           float x[V];
 #pragma unroll
           for (int k = 0; k < V; ++k) x[k] = acc_lds[sl * W + seg * V + k];
@@ -715,7 +720,7 @@ hipError_t launch_cs_prep(const CsArgs &a, hipStream_t s) {
 }
 
 size_t cs_lds_bytes(int32_t vec, int32_t cap) {
-  return (size_t)cap * kCsLanes * vec * sizeof(float);
+  return (size_t)cap * (kCsLanes * vec * sizeof(float) + sizeof(int32_t));
 }
 
 template <int DT, int V, bool kPacked>
