@@ -644,17 +644,19 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
     const int32_t e = tok ? (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile) : 0;
     const int32_t flags = tok ? a.tile_flags[t] : 0;
     const bool walk = (flags & 4) != 0;  // the same for the wave's tiles
-    // the rows of the interior slots, read once for all panels (the panel
-    // loop's first barrier orders these writes before the first write-out)
+    // the rows of the interior slots, read once for all panels
     int32_t *s_rows = reinterpret_cast<int32_t *>(acc_lds + (size_t)a.cap * W);
     for (int32_t i = threadIdx.x; i < n_slots - 2 * a.group; i += kCsThreads)
       s_rows[i] = a.slot_rows[(int64_t)unit * a.cap + 2 * a.group + i];
+    // slots start at zero; each panel's write-out zeroes what it read
+    for (int32_t i = threadIdx.x; i < n_slots * W / 4; i += kCsThreads)
+      reinterpret_cast<float4 *>(acc_lds)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    float z[V];
+    zero_vec(z);
     for (int32_t panel = 0; panel < panels; ++panel) {
       const int32_t col = panel * W + q * V;
       const bool colok = col < a.N;
-      for (int32_t i = threadIdx.x; i < n_slots * W / 4; i += kCsThreads)
-        reinterpret_cast<float4 *>(acc_lds)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      __syncthreads();
       if (walk)
         cs_walk<DT, V, kPacked>(a, t, s, e, col, colok);
       else
@@ -667,11 +669,13 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
 #pragma unroll
           for (int k = 0; k < V; ++k) x[k] = acc_lds[(2 * g) * W + q * V + k];
           store_vec<V>(a.carry_head + (int64_t)t * a.N + col, x);
+          store_vec<V>(acc_lds + (2 * g) * W + q * V, z);
         }
         if (flags & 2) {
 #pragma unroll
           for (int k = 0; k < V; ++k) x[k] = acc_lds[(2 * g + 1) * W + q * V + k];
           store_vec<V>(a.carry_tail + (int64_t)t * a.N + col, x);
+          store_vec<V>(acc_lds + (2 * g + 1) * W + q * V, z);
         }
       }
       // interior slots to C
@@ -687,6 +691,7 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
 #pragma unroll
           for (int k = 0; k < V; ++k) x[k] = acc_lds[sl * W + seg * V + k];
           store_vec<V>(a.C + (int64_t)row * a.ldc + c, x);
+          store_vec<V>(acc_lds + sl * W + seg * V, z);
         }
       }
       __syncthreads();
