@@ -326,7 +326,8 @@ def _check_preparation(pos, crd, vals, K, N, dtype="f32", **kw):
 
 
 @pytest.mark.parametrize("case", ["powerlaw", "banded", "one_column_in_every_row",
-                                  "one_entry_rows", "mixed", "wide", "no_values", "bf16"])
+                                  "one_entry_rows", "mixed", "wide", "no_values", "bf16",
+                                  "empty_runs", "one_row", "one_tile"])
 def test_preparation_arrays(case):
     """The preparation's arrays on structures that take each of its paths:
     the 8- and 16-key bucket sorts (power-law, banded), the whole-wave sort
@@ -357,6 +358,23 @@ def test_preparation_arrays(case):
         lens = np.concatenate([np.ones(60000, np.int64), rng.integers(20, 60, 6000)])
         pos, crd, vals, _ = _mk(rng, lens, 30000, 1)
         kw = dict(tile=128)
+    elif case == "empty_runs":
+        # empty rows first, last and in long runs inside tiles, and a row
+        # wider than the 64-row step of the row walk
+        lens = rng.integers(0, 12, 9000)
+        lens[:700] = 0
+        lens[3000:3400] = 0
+        lens[5000:5600:3] = 0
+        lens[-900:] = 0
+        lens[4000] = 2500
+        pos, crd, vals, _ = _mk(rng, lens, 30000, 1)
+        kw = dict(tile=256)
+    elif case == "one_row":  # every tile inside one row: heads and tails only
+        pos, crd, vals, _ = _mk(rng, np.array([20000]), 30000, 1)
+        kw = dict(tile=128)
+    elif case == "one_tile":  # fewer entries than one tile
+        pos, crd, vals, _ = _mk(rng, rng.integers(0, 5, 40), 30000, 1)
+        kw = dict(tile=512)
     else:  # wide
         K = (1 << 20) + 4099
         lens = np.concatenate([rng.integers(0, 30, 8000), np.ones(20000, np.int64)])
