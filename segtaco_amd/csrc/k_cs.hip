@@ -26,14 +26,16 @@
 //     written sorted by (column, position) as one record each (column, slot,
 //     value). A walking tile's entries stay in CSR order with the in-tile
 //     names;
-//   * k_eb_cs, per unit and per column panel of W = 8*V columns: zero the
-//     unit's LDS slots, let every sweeping lane group run its sorted tile —
-//     per entry one 16-byte gather per lane and one read-modify-write of its
-//     row's slot, with no branch on the kind of row — and every walking group
-//     run its tile in CSR order with register accumulators, written out at
-//     each change of row; then write interior slots to C and head / tail
-//     slots to the carry arrays. k_tile_rows_fix zeroes the empty rows and
-//     k_fixup combines the carries, as for eb_sr.
+//   * k_eb_cs, per unit: the rows of its interior slots into LDS and its
+//     slots zeroed, then per column panel of W = 8*V columns: every sweeping
+//     lane group runs its sorted tile — per entry one 16-byte gather per lane
+//     and one read-modify-write of its row's slot, with no branch on the kind
+//     of row — and every walking group runs its tile in CSR order with
+//     register accumulators, written out at each change of row; then interior
+//     slots go to C and head / tail slots to the carry arrays, each zeroed as
+//     it is read for the next panel. Block barriers around each panel's sweep
+//     keep the unit's waves on the same band of B rows. k_tile_rows_fix
+//     zeroes the empty rows and k_fixup combines the carries, as for eb_sr.
 // Each row's entries meet its accumulator in ascending column order (sweep)
 // or CSR order (walk), the same order for canonical CSR, with one fmaf per
 // entry starting from zero: C is bit-identical to eb_sr with the same tile
