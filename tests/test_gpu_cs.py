@@ -75,6 +75,27 @@ def test_long_rows_empty_runs_and_partial_panel():
     assert np.array_equal(C_cs.view(np.uint32), C_sr.view(np.uint32))
 
 
+@pytest.mark.parametrize("case", ["one_row", "one_tile", "leading_and_trailing_empty"])
+def test_bit_identical_edge_structures(case):
+    """A single row split over many tiles (heads and tails only), fewer
+    entries than one tile, and a matrix whose first and last thousand rows
+    are empty."""
+    rng = np.random.default_rng(17)
+    if case == "one_row":
+        lens, kw = np.array([20000]), dict(tile=128)
+    elif case == "one_tile":
+        lens, kw = rng.integers(0, 5, 40), dict(tile=512)
+    else:
+        lens = rng.integers(0, 30, 8000)
+        lens[:1000] = 0
+        lens[-1000:] = 0
+        kw = {}
+    pos, crd, vals, B = _mk(rng, lens, 30000, 40)
+    C_cs, C_sr, _ = _pair(pos, crd, vals, B, **kw)
+    assert _bound(pos, crd, vals, B, C_cs) <= TOL
+    assert np.array_equal(C_cs.view(np.uint32), C_sr.view(np.uint32))
+
+
 def test_lds_overflow_units_walk():
     """One entry per row: a unit of 64 tiles x 512 entries holds 32768 interior
     rows, far beyond the LDS accumulators (1216 slots at vec 4), so every wave
