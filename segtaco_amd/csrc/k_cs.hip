@@ -226,8 +226,9 @@ __global__ __launch_bounds__(kCsPrepThreads) void k_cs_count(const CsArgs a) {
   const int32_t r_hi = min(a.M - 1, a.tile_row[t + 1]);
   int32_t count = 0;
   bool head = false, tail = false;
-  for (int32_t rb = a.tile_row[t]; rb <= r_hi; rb += 64) {
-    const int32_t r = rb + lane;
+  // 64-bit row steps: rows near INT32_MAX must not wrap the loop
+  for (int64_t rb = a.tile_row[t]; rb <= r_hi; rb += 64) {
+    const int32_t r = (int32_t)min(rb + lane, (int64_t)r_hi + 1);
     int32_t p0 = e, p1 = e;  // past r_hi: as an empty row at the tile's end
     if (r <= r_hi) {
       p0 = a.pos[r];
