@@ -469,11 +469,13 @@ __device__ __forceinline__ float cs_widen(uint32_t r) {
 }
 
 // Entries per batch: metadata of batch b + 2, gathers of batch b + 1 and
-// the accumulation of batch b are in flight together. Wide records hold
-// three words, so their batches are smaller (all buffers stay in registers).
+// the accumulation of batch b are in flight together. Measured on the
+// power-law 262144^2 product at N = 128 (prepared run, us per call): 1: 408,
+// 2: 274, 3: 260, 4: 272, 5: 292, 6: 298, 7: 303, 8: 303 — small batches
+// keep each wave's gathers and LDS updates interleaved more finely.
 template <bool kPacked>
 constexpr int cs_batch() {
-  return kPacked ? 8 : 6;
+  return 3;
 }
 
 template <int NB>
