@@ -42,10 +42,13 @@ int32_t segtaco_row_stats(const int32_t *pos_host, int32_t M, int32_t K,
 int32_t segtaco_tune_model(const segtaco_row_stats_t *st, int32_t N,
                            int32_t b_dtype, segtaco_plan_t *out);
 
-/* Measured choice: times the model's plan and a candidate set around it
- * (reps timed runs each, after one warm-up) on `stream` with operands already
- * on the device, writes C, and returns the fastest plan and its mean ms.
- * The result is also stored in the plan cache under the signature. */
+/* Measured choice: times the model's plan and a candidate set around it,
+ * SEGTACO_EB_CS included (reps timed runs each, after one warm-up), on
+ * `stream` with operands already on the device, writes C with the fastest
+ * plan, and returns that plan and its mean ms. The plan cache, which batches
+ * and the server read, stores under the signature the fastest candidate they
+ * can run (SEGTACO_RB_SR .. SEGTACO_EB_PR), which differs from the returned
+ * plan when SEGTACO_EB_CS wins. */
 int32_t segtaco_tune_measure(const segtaco_csr_t *A,
                              const segtaco_row_stats_t *st, const void *B,
                              int32_t b_dtype, int64_t ldb, int32_t N, float *C,

@@ -772,7 +772,9 @@ def tune_cache_load(path: str) -> None:
 
 def tune_measure(A: CSR, st: RowStats, B_ptr: int, b_dtype: int, ldb: int, N: int, C_ptr: int,
                  ldc: int, reps: int = 3, stream: int | None = None):
-    """Times candidate plans on the device; returns (best plan, its ms)."""
+    """Times candidate plans on the device (eb_cs among them); returns (best
+    plan, its ms). The plan cache keeps the best plan batches and the server
+    can run (rb_sr .. eb_pr), which differs from the result when eb_cs wins."""
     p = Plan()
     ms = ctypes.c_float()
     check(lib().segtaco_tune_measure(ctypes.byref(A), ctypes.byref(st), B_ptr, b_dtype, ldb, N,

@@ -9,8 +9,10 @@
 //   * skewed (power-law) lengths: eb_sr — equal-entry tiles keep every lane
 //     group equally busy whatever the tail; tile 128 for short mean rows,
 //     256 above 32 entries per row.
-// The measured mode times the model's plan and a fixed candidate set and
-// caches the winner by a signature of the statistics.
+// The measured mode times the model's plan and a fixed candidate set
+// (eb_cs among them) and returns the fastest; the cache, keyed by a
+// signature of the statistics, keeps the fastest plan that batches and the
+// server can run (rb_sr .. eb_pr).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -54,6 +56,11 @@ segtaco_plan_t make(int32_t sched, int32_t vec, int32_t q, int32_t g,
   p.tile = tile;
   p.comp_threshold = -1;
   return p;
+}
+
+// Schedules that batches and the server run (eb_cs and bd run standalone).
+bool shared_schedule(int32_t sched) {
+  return sched >= SEGTACO_RB_SR && sched <= SEGTACO_EB_PR;
 }
 
 bool same_plan(const segtaco_plan_t &a, const segtaco_plan_t &b) {
@@ -171,7 +178,7 @@ int32_t segtaco_tune_cache_get(uint64_t sig, segtaco_plan_t *out) {
 int32_t segtaco_tune_cache_put(uint64_t sig, const segtaco_plan_t *plan) {
   if (!plan) return set_error(SEGTACO_ERR_INVALID, "null plan");
   // the cache feeds batches and the server, which run rb_sr .. eb_pr
-  if (plan->schedule < SEGTACO_RB_SR || plan->schedule > SEGTACO_EB_PR)
+  if (!shared_schedule(plan->schedule))
     return set_error(SEGTACO_ERR_UNSUPPORTED, "tune cache: schedule %d runs standalone only",
                      plan->schedule);
   std::lock_guard<std::mutex> lock(g_cache_mu);
@@ -208,7 +215,7 @@ int32_t segtaco_tune_cache_load(const char *path) {
   segtaco_plan_t p{};
   while (std::fscanf(f, "%llx %d %d %d %d %d %d", &sig, &p.schedule, &p.vec,
                      &p.col_lanes, &p.group, &p.tile, &p.comp_threshold) == 7) {
-    if (p.schedule < SEGTACO_RB_SR || p.schedule > SEGTACO_EB_PR) continue;
+    if (!shared_schedule(p.schedule)) continue;
     cache()[(uint64_t)sig] = p;
   }
   std::fclose(f);
@@ -245,6 +252,7 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
       make(SEGTACO_EB_SR, 0, 0, 0, 256),
       make(SEGTACO_EB_PR, 0, 0, 0, 0),
       make(SEGTACO_EB_PR, 0, 0, 0, 256),
+      make(SEGTACO_EB_CS, 0, 0, 0, 0),
   };
   std::vector<segtaco_plan_t> plans;
   size_t ws_need = 0;
@@ -263,8 +271,8 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
   hipEvent_t e0, e1;
   SEGTACO_HIP(hipEventCreate(&e0));
   SEGTACO_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  segtaco_plan_t best_plan = model;
+  float best = 1e30f, best_shared = 1e30f;
+  segtaco_plan_t best_plan = model, shared_plan = model;
   int32_t status = SEGTACO_OK;
   for (const auto &p : plans) {
     status = spmm_impl(A, B, b_dtype, ldb, N, C, ldc, &p, ws, ws_need, stream, nullptr);
@@ -285,6 +293,10 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
       best = ms;
       best_plan = p;
     }
+    if (shared_schedule(p.schedule) && ms < best_shared) {
+      best_shared = ms;
+      shared_plan = p;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -299,7 +311,7 @@ int32_t segtaco_tune_measure(const segtaco_csr_t *A,
   if (status != SEGTACO_OK) return status;
   *out = best_plan;
   if (out_ms) *out_ms = best;
-  segtaco_tune_cache_put(tune_signature_impl(st, N, b_dtype), &best_plan);
+  segtaco_tune_cache_put(tune_signature_impl(st, N, b_dtype), &shared_plan);
   return SEGTACO_OK;
 }
 

@@ -149,5 +149,35 @@ def test_autotune_measure_picks_a_correct_plan():
     err = O.scaled_err(C.cpu().numpy(), O.spmm_f64(pos, crd, vals, B),
                        O.spmm_absbound(pos, crd, vals, B))
     assert err <= 1e-5
-    assert S.tune(st, 64).as_dict() == plan.as_dict()  # cached under the signature
+    cached = S.tune(st, 64)  # cached under the signature
+    if plan.schedule == S.EB_CS:  # standalone only: the cache keeps a shared schedule
+        assert S.RB_SR <= cached.schedule <= S.EB_PR
+    else:
+        assert cached.as_dict() == plan.as_dict()
+    S.tune_cache_clear()
+
+
+def test_autotune_measure_offers_eb_cs_to_standalone_calls():
+    """On the power-law 262144^2 product (4.2 M entries) at N = 128, FP32,
+    eb_cs's step is about a fifth faster than every other candidate's; the
+    measured choice returns it, leaves C correct, and caches a plan that
+    batches can run."""
+    M = 262144
+    pos, crd, vals = S.gen_csr(S.GEN_POWERLAW, 43, M, M)
+    B = S.gen_dense(44, M, 128)
+    A = DeviceCSR(pos, crd, vals, M)
+    Bd = dense_dev(B)
+    C = torch().full((M, 128), float("nan"), dtype=torch().float32, device="cuda")
+    st = S.row_stats(pos, M)
+    S.tune_cache_clear()
+    plan, ms = S.tune_measure(A.desc, st, Bd.data_ptr(), S.DT_F32, 128, 128, C.data_ptr(), 128,
+                              reps=3, stream=stream_ptr())
+    torch().cuda.synchronize()
+    print(f"[tune] nnz {int(pos[-1])} chose {plan} at {ms:.4f} ms", flush=True)
+    assert plan.schedule == S.EB_CS
+    err = O.scaled_err(C.cpu().numpy(), O.spmm_f64(pos, crd, vals, B),
+                       O.spmm_absbound(pos, crd, vals, B))
+    assert err <= 1e-5
+    cached = S.tune(st, 128)
+    assert S.RB_SR <= cached.schedule <= S.EB_PR
     S.tune_cache_clear()
