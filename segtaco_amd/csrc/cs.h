@@ -57,6 +57,19 @@ struct CsArgs {
                         // the tile walks
 };
 
+// Whether k_eb_cs writes the row that spans tiles t0 < t1 itself, summing
+// its LDS head / tail slots in the fix-up's order (tail of t0, then the heads
+// of t0 + 1 .. t1), instead of leaving it to the carries and k_fixup: both
+// tiles in one unit of `group` tiles, and none of t0 .. t1 walking (flags:
+// bit 2 per walking tile).
+__device__ inline bool cs_row_folded(const uint8_t *flags, int32_t group, int32_t t0,
+                                     int32_t t1) {
+  if (t0 / group != t1 / group) return false;
+  for (int32_t t = t0; t <= t1; ++t)
+    if (flags[t] & 4) return false;
+  return true;
+}
+
 hipError_t launch_cs_prep(const CsArgs &a, hipStream_t s);
 hipError_t launch_eb_cs(int dt, int vec, const CsArgs &a, hipStream_t s);
 size_t cs_lds_bytes(int32_t vec, int32_t cap);

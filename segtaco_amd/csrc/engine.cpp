@@ -588,7 +588,10 @@ int32_t spmm_impl(const segtaco_csr_t *A, const void *B, int32_t b_dtype,
     c.n_slots = reinterpret_cast<int32_t *>(w + L.nslot_off);
     c.tile_count = reinterpret_cast<int32_t *>(w + L.count_off);
     c.tile_flags = reinterpret_cast<uint8_t *>(w + L.flag_off);
-    // the fix-up runs with its own natural lane count for N
+    // the fix-up runs with its own natural lane count for N, and skips the
+    // rows k_eb_cs wrote itself
+    a.cs_flags = c.tile_flags;
+    a.cs_group = c.group;
     a.lq = ilog2(natural_lanes(N, V));
     a.panels = (int32_t)ceil_div(N, (int64_t)(1 << a.lq) * V);
     // The tile rows are rewritten on every call: the same kernel zeroes C's

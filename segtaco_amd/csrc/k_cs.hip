@@ -32,10 +32,13 @@
 //     and one read-modify-write of its row's slot, with no branch on the kind
 //     of row — and every walking group runs its tile in CSR order with
 //     register accumulators, written out at each change of row; then interior
-//     slots go to C and head / tail slots to the carry arrays, each zeroed as
-//     it is read for the next panel. Block barriers around each panel's sweep
-//     keep the unit's waves on the same band of B rows. k_tile_rows_fix
-//     zeroes the empty rows and k_fixup combines the carries, as for eb_sr.
+//     slots go to C, a row whose tiles all sweep in the unit is summed from
+//     its head / tail slots by the tile where it ends (in k_fixup's order),
+//     and the other head / tail slots go to the carry arrays; each slot is
+//     zeroed as it is read, for the next panel. Block barriers around each
+//     panel's sweep keep the unit's waves on the same band of B rows.
+//     k_tile_rows_fix zeroes the empty rows and k_fixup combines the
+//     remaining carries, as for eb_sr.
 // Each row's entries meet its accumulator in ascending column order (sweep)
 // or CSR order (walk), the same order for canonical CSR, with one fmaf per
 // entry starting from zero: C is bit-identical to eb_sr with the same tile
@@ -649,6 +652,27 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
     const int32_t e = tok ? (int32_t)min((int64_t)a.nnz, (int64_t)s + a.tile) : 0;
     const int32_t flags = tok ? a.tile_flags[t] : 0;
     const bool walk = (flags & 4) != 0;  // the same for the wave's tiles
+    // A sweeping tile's head and tail rows go to the carries unless the
+    // unit writes them itself (cs_row_folded): then the tile where such a
+    // row ends sums its slots after each panel, in the fix-up's order.
+    bool head_carry = false, tail_carry = false;
+    int32_t fold_t0 = -1, fold_row = 0;
+    if (tok && !walk) {
+      if (flags & 1) {  // row holding entry t * T, begun in an earlier tile
+        const int32_t r = a.tile_row[t];
+        const int32_t t0 = a.pos[r] / a.tile, t1 = (a.pos[r + 1] - 1) / a.tile;
+        if (!cs_row_folded(a.tile_flags, a.group, t0, t1)) {
+          head_carry = true;
+        } else if (t1 == t) {
+          fold_t0 = t0;
+          fold_row = r;
+        }
+      }
+      if (flags & 2) {  // row holding entry (t + 1) * T, begun in this tile
+        const int32_t r = a.tile_row[t + 1];
+        tail_carry = !cs_row_folded(a.tile_flags, a.group, t, (a.pos[r + 1] - 1) / a.tile);
+      }
+    }
     // the rows of the interior slots, read once for all panels
     int32_t *s_rows = reinterpret_cast<int32_t *>(acc_lds + (size_t)a.cap * W);
     for (int32_t i = threadIdx.x; i < n_slots - 2 * a.group; i += kCsThreads)
@@ -667,20 +691,36 @@ __global__ __launch_bounds__(kCsThreads) void k_eb_cs(const CsArgs a) {
       else
         cs_sweep<DT, V, kPacked>(a, acc_lds, s, e, col, colok);
       __syncthreads();
-      // head / tail slots of sweeping tiles to the carries
+      // head / tail slots of sweeping tiles to the carries, or rows that
+      // end in this tile summed from the unit's slots (each slot is read
+      // and zeroed by one lane group: its own, or the group that sums it)
       if (tok && colok && !walk) {
         float x[V];
-        if (flags & 1) {
+        if (head_carry) {
 #pragma unroll
           for (int k = 0; k < V; ++k) x[k] = acc_lds[(2 * g) * W + q * V + k];
           store_vec<V>(a.carry_head + (int64_t)t * a.N + col, x);
           store_vec<V>(acc_lds + (2 * g) * W + q * V, z);
         }
-        if (flags & 2) {
+        if (tail_carry) {
 #pragma unroll
           for (int k = 0; k < V; ++k) x[k] = acc_lds[(2 * g + 1) * W + q * V + k];
           store_vec<V>(a.carry_tail + (int64_t)t * a.N + col, x);
           store_vec<V>(acc_lds + (2 * g + 1) * W + q * V, z);
+        }
+        if (fold_t0 >= 0) {
+          const int32_t u0 = unit * a.group;
+          float *sl = acc_lds + (2 * (fold_t0 - u0) + 1) * W + q * V;
+#pragma unroll
+          for (int k = 0; k < V; ++k) x[k] = sl[k];
+          store_vec<V>(sl, z);
+          for (int32_t tt = fold_t0 + 1; tt <= t; ++tt) {
+            sl = acc_lds + (2 * (tt - u0)) * W + q * V;
+#pragma unroll
+            for (int k = 0; k < V; ++k) x[k] += sl[k];
+            store_vec<V>(sl, z);
+          }
+          store_vec<V>(a.C + (int64_t)fold_row * a.ldc + col, x);
         }
       }
       // interior slots to C

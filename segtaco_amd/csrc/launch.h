@@ -35,6 +35,11 @@ struct KArgs {
   // row began in an earlier tile, else -1 (written by k_tile_rows_fix, which
   // also zeroes the empty rows of C); null: the fix-up derives both from pos
   const int32_t *fix_t0;
+  // eb_cs: per tile bit 2 = the tile walks (k_cs_sort), and tiles per unit.
+  // The rows that cs_row_folded names were written by k_eb_cs itself, so the
+  // fix-up skips them. Null / 0 for the other schedules.
+  const uint8_t *cs_flags;
+  int32_t cs_group;
   float *carry_head;       // eb_*: [ntiles][N] partial of a continued row
   float *carry_tail;       // eb_*: [ntiles][N] partial of a row left open
   // K, ldb < 2^24 and every byte offset into B fits 32 bits: gathers form

@@ -43,6 +43,7 @@
 
 #include <climits>
 
+#include "cs.h"
 #include "device_common.h"
 
 namespace segtaco {
@@ -502,8 +503,10 @@ __device__ __forceinline__ void fixup_body(const KArgs &a, int32_t blk,
   const int32_t col = ((panel << lq) + q) * V;
   if (a.fix_t0) {
     const int32_t t0 = a.fix_t0[t];
-    const int32_t r = a.tile_row[t];
-    if (t0 >= 0 && col < a.N) combine_carries<V>(a, t0, t, r, col);
+    if (t0 < 0 || col >= a.N) return;
+    // eb_cs has already written the rows it could sum from its LDS slots
+    if (a.cs_flags && cs_row_folded(a.cs_flags, a.cs_group, t0, t)) return;
+    combine_carries<V>(a, t0, t, a.tile_row[t], col);
     return;
   }
   {
