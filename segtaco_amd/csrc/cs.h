@@ -13,7 +13,10 @@ constexpr int kCsPrepThreads = 256;       // prep kernels: one tile per wave
 constexpr int32_t kCsLanes = 8;           // lanes per entry (Q)
 constexpr int32_t kCsMaxGroup = kCsWaves * (64 / kCsLanes);  // tiles per unit
 constexpr int32_t kCsMaxTile = 512;       // entries per tile
-constexpr size_t kCsLdsBudget = 152 * 1024;  // accumulator bytes per unit
+// Accumulator bytes per unit: a whole CU's LDS on gfx950. The slots' row ids
+// (4 bytes each) share it, so the slot count is also capped at the device's
+// LDS / (slot bytes + 4): 1241 slots of 128 bytes.
+constexpr size_t kCsLdsBudget = 160 * 1024;
 // One record per sorted entry, read with one load: {(column << kCsCodeBits)
 // | slot, value bits} in 64 bits when every column is below
 // 2^(32 - kCsCodeBits) (slots are below 2^kCsCodeBits), otherwise {column,

@@ -98,7 +98,7 @@ def test_bit_identical_edge_structures(case):
 
 def test_lds_overflow_units_walk():
     """One entry per row: a unit of 64 tiles x 512 entries holds 32768 interior
-    rows, far beyond the LDS accumulators (1216 slots at vec 4), so every wave
+    rows, far beyond the LDS accumulators (1241 slots at vec 4), so every wave
     walks its tiles in CSR order; results are unchanged."""
     rng = np.random.default_rng(4)
     M = 70000
