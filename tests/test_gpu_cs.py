@@ -113,7 +113,7 @@ def test_lds_overflow_units_walk():
 
 def test_units_with_sweeping_and_walking_waves():
     """Short rows first, long rows after: with 128-entry tiles a wave of 8
-    tiles over one-entry rows holds 1024 interior rows, which fit the 1088
+    tiles over one-entry rows holds 1024 interior rows, which fit the 1113
     interior slots once, so in those units wave 0 sweeps and the rest walk;
     the units over long rows all sweep, and the unit at the boundary mixes
     both kinds of wave."""
