@@ -59,11 +59,15 @@ __device__ __forceinline__ void dispatch(int32_t sched, int32_t vec,
   }
 }
 
-// FP32: held at 4 waves per SIMD (128 VGPRs), which every FP32 body fits;
-// the 16-bit forms need more registers and run at 3.
+// Held at 4 waves per SIMD (128 VGPRs). Every FP32 body fits; the 16-bit
+// forms would take 145 VGPRs (3 waves) and instead spill 48-60 bytes per
+// lane. Measured in BF16 on the 512-matrix power-law pool, 3 waves -> 4:
+// server p50 275 -> 252 us, p99 908 -> 866 us, saturated 4668 -> 4828
+// GFLOP/s, power-law N = 128 burst 5701 -> 5964 GFLOP/s; the 512-product
+// launch 57.3 -> 58.8 ms.
 template <int DT, bool kSv>
 __global__ __launch_bounds__(kBlockThreads)
-__attribute__((amdgpu_waves_per_eu(DT == 0 ? 4 : 1))) void k_batch_main(
+__attribute__((amdgpu_waves_per_eu(4))) void k_batch_main(
     const BatchDesc *__restrict__ d, int32_t n,
     const int64_t *__restrict__ prefix, int64_t total,
     unsigned long long *counter) {
