@@ -233,9 +233,8 @@ int32_t resolve(const segtaco_csr_t *A, int32_t N, int32_t b_dtype,
       if (plan->col_lanes == 0) plan->col_lanes = kCsLanes;
       if (plan->group == 0) plan->group = kCsMaxGroup;
       if (plan->tile == 0) {
-        // one unit per CU: nnz / (CUs * tiles per unit), a multiple of 8
-        const int64_t per = ceil_div((int64_t)A->nnz, (int64_t)device_cus() * plan->group);
-        int64_t t = ceil_div(per, 8) * 8;
+        // one unit per CU: nnz / (CUs * tiles per unit), rounded up
+        int64_t t = ceil_div((int64_t)A->nnz, (int64_t)device_cus() * plan->group);
         if (t < 32) t = 32;
         if (t > kCsMaxTile) t = kCsMaxTile;
         plan->tile = (int32_t)t;

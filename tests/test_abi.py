@@ -187,8 +187,9 @@ def test_generators_deterministic_and_shaped():
 def test_eb_cs_plan_resolution():
     A = _desc(M=262144, K=262144, nnz=4204209)
     p = S.plan_resolve(A, 128, S.DT_F32, S.make_plan("eb_cs"))
-    # one 64-tile unit per compute unit (256 on MI355X): 4204209 / 16384 -> 264
-    assert (p.vec, p.col_lanes, p.group, p.tile, p.comp_threshold) == (4, 8, 64, 264, 0)
+    # one 64-tile unit per compute unit (256 on MI355X): 4204209 / 16384 -> 257,
+    # 16359 tiles in 256 units
+    assert (p.vec, p.col_lanes, p.group, p.tile, p.comp_threshold) == (4, 8, 64, 257, 0)
     assert S.workspace_bytes(A, 128, p) > S.workspace_bytes(A, 128, S.plan_resolve(
         A, 128, S.DT_F32, S.make_plan("eb_sr", tile=264)))
     # 16-bit storage: vector capped at 4; eb_cs never compensates (the
