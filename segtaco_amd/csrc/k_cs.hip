@@ -32,7 +32,7 @@
 //     and one read-modify-write of its row's slot, with no branch on the kind
 //     of row — and every walking group runs its tile in CSR order with
 //     register accumulators, written out at each change of row; then interior
-//     slots go to C, a row whose tiles all sweep in the unit is summed from
+//     slots go to C, a row over two neighbouring sweeping tiles is summed from
 //     its head / tail slots by the tile where it ends (in k_fixup's order),
 //     and the other head / tail slots go to the carry arrays; each slot is
 //     zeroed as it is read, for the next panel. Block barriers around each
